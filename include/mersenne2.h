@@ -1,0 +1,114 @@
+/*
+ * mersenne2 on MI355X: C ABI of the IBDWT squaring engine for Lucas-Lehmer and
+ * base-3 PRP tests of 2^p - 1 (libm2.so).
+ *
+ * The reference (cherubrock-seb/mersenne2GPU) has no library interface: its
+ * `mersenne2` program owns the whole test inside main() (mersenne2_gpu.cpp:
+ * 3618-4960). Each entry point below replaces one part of that program, cited
+ * per function; the `mersenne2` command line of this project is built on them.
+ *
+ * Conventions: every function returns 0 (M2_OK) or a negative M2_E* code and
+ * never throws; m2_last_error() gives the message of the calling thread's last
+ * failure. Plain pointers and sizes only. A test handle is used by one thread
+ * at a time; a context may create and destroy tests from several threads.
+ * Residues are canonical little-endian integers of ceil(p/8) bytes, 0 <= R <
+ * 2^p - 1; Res64 is R mod 2^64. "Iteration k" is the state after k squarings
+ * (PRP, from 3) or k Lucas-Lehmer steps (LL, from 4).
+ */
+#ifndef MERSENNE2_H
+#define MERSENNE2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M2_OK 0
+#define M2_E_INVAL (-1)     /* bad argument (usage error; the CLI exits 1) */
+#define M2_E_DEVICE (-2)    /* no such device / no GPU */
+#define M2_E_NOMEM (-3)     /* device or host allocation failed */
+#define M2_E_HIP (-4)       /* a HIP runtime call failed */
+#define M2_E_RANGE (-5)     /* exponent outside the supported range */
+#define M2_E_STATE (-6)     /* call not valid in the object's current state */
+
+#define M2_MODE_LL 0
+#define M2_MODE_PRP 1
+
+typedef struct m2_ctx m2_ctx;
+typedef struct m2_test m2_test;
+typedef struct m2_batch m2_batch;
+
+/* Tuning options; zero fields mean "automatic". The reference's -W, -CW, -B
+ * and -LS flags (mersenne2_gpu.cpp:3660-3681) map onto these. */
+typedef struct m2_opts {
+  uint32_t wg;              /* -W: work-group size hint */
+  uint32_t carry_wg;        /* -CW: carry work-group size hint */
+  uint32_t carry_pairs;     /* -B: carry block size hint in element pairs */
+  uint32_t local_stage_cap; /* -LS: cap on LDS-staged transform points */
+  uint32_t graph_iters;     /* iterations per captured hipGraph (0: automatic) */
+  uint32_t reserved[3];
+} m2_opts;
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *m2_last_error(void);
+
+/* Number of visible HIP devices (the reference's GPU enumeration,
+ * mersenne2_gpu.cpp:3735-3758). */
+int m2_device_count(int *count);
+
+/* Open device `device` (-d, mersenne2_gpu.cpp:3646-3651, :3753-3764). */
+int m2_open(int device, m2_ctx **out);
+int m2_close(m2_ctx *ctx);
+/* Device name and properties (CL_DEVICE_NAME etc., mersenne2_gpu.cpp:3759-3770). */
+int m2_device_name(m2_ctx *ctx, char *buf, size_t len);
+int m2_device_props(m2_ctx *ctx, uint32_t *compute_units, uint64_t *global_mem, uint32_t *max_wg,
+                    uint32_t *lds_bytes);
+
+/* log2 of the transform length for p (transformsize, mersenne2_gpu.cpp:286-293). */
+int m2_transform_log2(uint64_t p, int *ln);
+/* Device bytes one test of exponent p holds while resident. */
+int m2_test_footprint(uint64_t p, uint64_t *bytes);
+
+/* Create a test at iteration 0 (tables, buffers and initial residue:
+ * mersenne2_gpu.cpp:3902-4028). mode: M2_MODE_LL or M2_MODE_PRP. */
+int m2_test_create(m2_ctx *ctx, uint64_t p, int mode, const m2_opts *opts, m2_test **out);
+int m2_test_destroy(m2_test *t);
+
+/* Enqueue `iters` iterations on the test's stream and return (the hot loop,
+ * mersenne2_gpu.cpp:4811-4853). */
+int m2_test_run(m2_test *t, uint64_t iters);
+/* Wait for the test's stream (clFinish, mersenne2_gpu.cpp:4788, :4855). */
+int m2_test_sync(m2_test *t);
+int m2_test_iteration(m2_test *t, uint64_t *iter);
+/* Iterations of a full test: p (PRP) or p - 2 (LL) (mersenne2_gpu.cpp:4780). */
+int m2_test_total_iters(m2_test *t, uint64_t *iters);
+
+/* Res64 and full residue of the current state (synchronises). The reference
+ * prints neither; it reads the residue back (mersenne2_gpu.cpp:4862) only to
+ * test it. */
+int m2_test_res64(m2_test *t, uint64_t *res64);
+int m2_test_residue(m2_test *t, uint8_t *buf, size_t len);
+/* Replace the state by residue R at iteration `iteration` (resume). */
+int m2_test_set_residue(m2_test *t, const uint8_t *buf, size_t len, uint64_t iteration);
+/* Verdict of the current state (verify_is_zero / verify_is_Mp /
+ * verify_prp_residue_9, mersenne2_gpu.cpp:3028-3076, :4865-4867):
+ * 1 if LL residue is 0 or PRP residue is 9 (mod 2^p - 1), else 0. */
+int m2_test_is_prime(m2_test *t, int *verdict);
+
+/* Run `iters` iterations and return their wall time on the device in ms
+ * (hipEvents around the launches). */
+int m2_test_time(m2_test *t, uint64_t iters, double *ms);
+/* Run `iters` iterations kernel by kernel with an event between kernels and
+ * accumulate per-kernel device time. names_buf receives the kernel names
+ * separated by '\n'; ms[i] the total milliseconds of kernel i. */
+int m2_test_kernel_times(m2_test *t, uint64_t iters, double *ms, int max_kernels, int *count, char *names_buf,
+                         size_t names_len);
+/* Human-readable plan (the reference's --debug-plan, mersenne2_gpu.cpp:4767-4776). */
+int m2_test_plan(m2_test *t, char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

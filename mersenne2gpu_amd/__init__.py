@@ -1,0 +1,244 @@
+"""mersenne2 on MI355X: Python view of the libm2 C ABI (include/mersenne2.h).
+
+The engine itself is C++/HIP (libm2.so, built in-tree by
+mersenne2gpu_amd.build); this module only binds it with ctypes so that tests,
+the benchmark and the serving loop can drive it. There is no fallback: if
+libm2.so is missing, importing `lib()` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, List, Optional, Sequence, Tuple
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libm2.so")
+CLI_PATH = os.path.join(HERE, "mersenne2")
+
+M2_OK = 0
+MODE_LL = 0
+MODE_PRP = 1
+ERRORS = {-1: "M2_E_INVAL", -2: "M2_E_DEVICE", -3: "M2_E_NOMEM", -4: "M2_E_HIP", -5: "M2_E_RANGE",
+          -6: "M2_E_STATE"}
+
+# Every entry point of include/mersenne2.h with its ctypes signature.
+_c = ctypes
+_P = _c.c_void_p
+SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
+    "m2_last_error": (_c.c_char_p, []),
+    "m2_device_count": (_c.c_int, [_c.POINTER(_c.c_int)]),
+    "m2_open": (_c.c_int, [_c.c_int, _c.POINTER(_P)]),
+    "m2_close": (_c.c_int, [_P]),
+    "m2_device_name": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
+    "m2_device_props": (_c.c_int, [_P, _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint32),
+                                   _c.POINTER(_c.c_uint32)]),
+    "m2_transform_log2": (_c.c_int, [_c.c_uint64, _c.POINTER(_c.c_int)]),
+    "m2_test_footprint": (_c.c_int, [_c.c_uint64, _c.POINTER(_c.c_uint64)]),
+    "m2_test_create": (_c.c_int, [_P, _c.c_uint64, _c.c_int, _P, _c.POINTER(_P)]),
+    "m2_test_destroy": (_c.c_int, [_P]),
+    "m2_test_run": (_c.c_int, [_P, _c.c_uint64]),
+    "m2_test_sync": (_c.c_int, [_P]),
+    "m2_test_iteration": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
+    "m2_test_total_iters": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
+    "m2_test_res64": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
+    "m2_test_residue": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
+    "m2_test_set_residue": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t, _c.c_uint64]),
+    "m2_test_is_prime": (_c.c_int, [_P, _c.POINTER(_c.c_int)]),
+    "m2_test_time": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double)]),
+    "m2_test_kernel_times": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double), _c.c_int, _c.POINTER(_c.c_int),
+                                        _c.c_char_p, _c.c_size_t]),
+    "m2_test_plan": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
+}
+
+_LIB = None
+
+
+class M2Error(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = ""
+        if _LIB is not None:
+            raw = _LIB.m2_last_error()
+            msg = raw.decode() if raw else ""
+        super().__init__("%s: %s (%s)" % (where, ERRORS.get(code, code), msg))
+        self.code = code
+
+
+def lib():
+    """The loaded libm2.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libm2.so not built: run `python -m mersenne2gpu_amd.build` (%s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _chk(rc: int, where: str) -> None:
+    if rc != M2_OK:
+        raise M2Error(rc, where)
+
+
+def transform_log2(p: int) -> int:
+    v = ctypes.c_int(0)
+    _chk(lib().m2_transform_log2(p, ctypes.byref(v)), "m2_transform_log2")
+    return v.value
+
+
+def footprint(p: int) -> int:
+    v = ctypes.c_uint64(0)
+    _chk(lib().m2_test_footprint(p, ctypes.byref(v)), "m2_test_footprint")
+    return v.value
+
+
+def device_count() -> int:
+    v = ctypes.c_int(0)
+    _chk(lib().m2_device_count(ctypes.byref(v)), "m2_device_count")
+    return v.value
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _chk(lib().m2_open(device, ctypes.byref(h)), "m2_open")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().m2_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _chk(lib().m2_device_name(self.h, buf, 256), "m2_device_name")
+        return buf.value.decode()
+
+    def props(self) -> Dict[str, int]:
+        cu, mem, wg, lds = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_uint32()
+        _chk(lib().m2_device_props(self.h, ctypes.byref(cu), ctypes.byref(mem), ctypes.byref(wg), ctypes.byref(lds)),
+             "m2_device_props")
+        return {"compute_units": cu.value, "global_mem": mem.value, "max_wg": wg.value, "lds_bytes": lds.value}
+
+    def test(self, p: int, mode: str = "prp", graph_iters: int = 0) -> "Test":
+        return Test(self, p, mode, graph_iters)
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [("wg", ctypes.c_uint32), ("carry_wg", ctypes.c_uint32), ("carry_pairs", ctypes.c_uint32),
+                ("local_stage_cap", ctypes.c_uint32), ("graph_iters", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+
+class Test:
+    """One LL or PRP test resident on the device (mirror of m2_test_*)."""
+
+    def __init__(self, ctx: Context, p: int, mode: str = "prp", graph_iters: int = 0):
+        self.p = int(p)
+        self.mode = mode
+        m = MODE_PRP if mode == "prp" else MODE_LL
+        opts = _Opts()
+        opts.graph_iters = graph_iters
+        h = ctypes.c_void_p()
+        _chk(lib().m2_test_create(ctx.h, self.p, m, ctypes.byref(opts), ctypes.byref(h)), "m2_test_create")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().m2_test_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, iters: int) -> None:
+        _chk(lib().m2_test_run(self.h, int(iters)), "m2_test_run")
+
+    def sync(self) -> None:
+        _chk(lib().m2_test_sync(self.h), "m2_test_sync")
+
+    @property
+    def iteration(self) -> int:
+        v = ctypes.c_uint64()
+        _chk(lib().m2_test_iteration(self.h, ctypes.byref(v)), "m2_test_iteration")
+        return v.value
+
+    @property
+    def total_iters(self) -> int:
+        v = ctypes.c_uint64()
+        _chk(lib().m2_test_total_iters(self.h, ctypes.byref(v)), "m2_test_total_iters")
+        return v.value
+
+    def res64(self) -> str:
+        v = ctypes.c_uint64()
+        _chk(lib().m2_test_res64(self.h, ctypes.byref(v)), "m2_test_res64")
+        return "%016X" % v.value
+
+    def residue(self) -> bytes:
+        n = (self.p + 7) // 8
+        buf = ctypes.create_string_buffer(n)
+        _chk(lib().m2_test_residue(self.h, buf, n), "m2_test_residue")
+        return buf.raw[:n]
+
+    def set_residue(self, r: bytes, iteration: int) -> None:
+        _chk(lib().m2_test_set_residue(self.h, r, len(r), int(iteration)), "m2_test_set_residue")
+
+    def is_prime(self) -> bool:
+        v = ctypes.c_int()
+        _chk(lib().m2_test_is_prime(self.h, ctypes.byref(v)), "m2_test_is_prime")
+        return bool(v.value)
+
+    def time(self, iters: int) -> float:
+        """Device milliseconds for `iters` iterations (hipEvents)."""
+        v = ctypes.c_double()
+        _chk(lib().m2_test_time(self.h, int(iters), ctypes.byref(v)), "m2_test_time")
+        return v.value
+
+    def kernel_times(self, iters: int) -> Dict[str, float]:
+        """Total device milliseconds per kernel over `iters` iterations."""
+        ms = (ctypes.c_double * 8)()
+        cnt = ctypes.c_int()
+        names = ctypes.create_string_buffer(512)
+        _chk(lib().m2_test_kernel_times(self.h, int(iters), ms, 8, ctypes.byref(cnt), names, 512),
+             "m2_test_kernel_times")
+        keys = names.value.decode().split("\n")[: cnt.value]
+        out: Dict[str, float] = {}
+        for k, v in zip(keys, ms[: cnt.value]):
+            out[k] = out.get(k, 0.0) + v
+        return out
+
+    def plan(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        _chk(lib().m2_test_plan(self.h, buf, 4096), "m2_test_plan")
+        return buf.value.decode()
+
+
+def run_checkpoints(ctx: Context, p: int, mode: str, checkpoints: Sequence[int]) -> Dict[int, str]:
+    """Res64 after each checkpoint iteration."""
+    out: Dict[int, str] = {}
+    with Test(ctx, p, mode) as t:
+        done = 0
+        for k in sorted(set(int(k) for k in checkpoints)):
+            t.run(k - done)
+            done = k
+            out[k] = t.res64()
+    return out

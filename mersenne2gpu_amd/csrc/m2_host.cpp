@@ -1,0 +1,373 @@
+// Host side of the engine: transform size, digit tables, twiddle tables, the
+// pass plan, and exact residue arithmetic on the host.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <sstream>
+
+#include "m2_internal.h"
+
+namespace m2 {
+
+// Smallest ln >= 3 with ln + 2((p >> ln) + 1) < 92, so that n (2^(q+1))^2 stays
+// below M61*M31 (the reference's transformsize, mersenne2_gpu.cpp:286-293).
+int transform_log2(uint64_t p) {
+  int ln = 3;
+  while ((uint64_t)ln + 2u * ((p >> ln) + 1u) >= 92u) ++ln;
+  return ln;
+}
+
+namespace {
+
+// 2-Sylow generators: order 2^62 in GF(M61^2), 2^32 in GF(M31^2). They are the
+// reference's constants (mersenne2_gpu.cpp:174, :209), kept so that spectra
+// match the oracle element by element.
+const e61 G61 = {264036120304204ull, 4677669021635377ull};
+const e31 G31 = {7735u, 748621u};
+
+template <class E>
+E gpow(E x, uint64_t e) {
+  E r;
+  r.a = 1;
+  r.b = 0;
+  while (e) {
+    if (e & 1) r = mul(r, x);
+    x = mul(x, x);
+    e >>= 1;
+  }
+  return r;
+}
+e61 root61(int k) { return gpow(G61, uint64_t(1) << (62 - k)); }  // order 2^k
+e31 root31(int k) { return gpow(G31, uint64_t(1) << (32 - k)); }
+
+uint32_t bitrev(uint32_t j, int bits) {
+  uint32_t r = 0;
+  for (int b = 0; b < bits; ++b) r |= ((j >> b) & 1u) << (bits - 1 - b);
+  return r;
+}
+
+int log2i(uint32_t r) { return r == 8 ? 3 : (r == 4 ? 2 : 1); }
+
+// Levels [ell0, ell0 + A): one radix-2 or radix-4 step for A mod 3 levels,
+// then radix-8 steps. Twiddle blocks are appended to T.
+Pass make_pass(int ell0, int A, HostTables& T) {
+  Pass ps{};
+  ps.ell0 = ell0;
+  ps.A = A;
+  int ell = ell0;
+  auto add_step = [&](int radix) {
+    Step& s = ps.steps[ps.nsteps++];
+    s.ell = ell;
+    s.radix = radix;
+    s.twoff = (uint32_t)T.tw61.size();
+    const int lr = log2i(radix);
+    // tau_j = zeta of block 2^lr * j at level ell + lr = w_{2^(ell+lr)}^bitrev(j)
+    const uint32_t nb = 1u << ell;
+    const e61 w61 = root61(ell + lr);
+    const e31 w31 = root31(ell + lr);
+    std::vector<e61> p61(nb);
+    std::vector<e31> p31(nb);
+    e61 c61{1, 0};
+    e31 c31{1, 0};
+    for (uint32_t k = 0; k < nb; ++k) {
+      p61[k] = c61;
+      p31[k] = c31;
+      c61 = mul(c61, w61);
+      c31 = mul(c31, w31);
+    }
+    for (uint32_t j = 0; j < nb; ++j) {
+      const uint32_t br = bitrev(j, ell);
+      const e61 t61 = p61[br];
+      const e31 t31 = p31[br];
+      e61 a61 = t61;
+      e31 a31 = t31;
+      for (int e = 1; e < radix; ++e) {
+        T.tw61.push_back(a61);
+        T.tw31.push_back(a31);
+        a61 = mul(a61, t61);
+        a31 = mul(a31, t31);
+      }
+    }
+    ell += lr;
+  };
+  const int rem = A % 3;
+  if (rem == 1) add_step(2);
+  if (rem == 2) add_step(4);
+  while (ell < ell0 + A) add_step(8);
+  return ps;
+}
+
+}  // namespace
+
+bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T, std::string& err) {
+  const int ln = transform_log2(p);
+  if (ln > 27) {
+    err = "exponent too large: transform length above 2^27";
+    return false;
+  }
+  const uint64_t n = uint64_t(1) << ln, h = n / 2;
+  const uint32_t q = p >> ln;
+  P.ln = ln;
+  P.L = ln - 1;
+  P.h = (uint32_t)h;
+  P.mode = mode;
+
+  // Digit widths ceil(p(j+1)/n) - ceil(pj/n) and weight shifts
+  // lr*(n - (pj mod n)) with 2^lr an n-th root of 2 in each field
+  // (mersenne2_gpu.cpp:3921-3944).
+  T.dig.assign(n, DigitInfo{0, 0, 0, 0});
+  T.bitpos.assign(n + 1, 0);
+  const uint64_t lr61 = ((uint64_t(1) << 60) >> ln) % 61u;
+  const uint64_t lr31 = ((uint64_t(1) << 30) >> ln) % 31u;
+  for (uint64_t j = 0; j <= n; ++j) T.bitpos[j] = j ? (uint32_t)(((uint64_t)p * j - 1) / n + 1) : 0u;
+  for (uint64_t j = 0; j < n; ++j) {
+    const uint32_t wd = T.bitpos[j + 1] - T.bitpos[j];
+    if (wd != q && wd != q + 1) {
+      err = "digit width generation error";
+      return false;
+    }
+    T.dig[j].width = (uint8_t)wd;
+    if (j) {
+      const uint64_t r = ((uint64_t)p * j) & (n - 1);
+      T.dig[j].w61 = (uint8_t)((lr61 * (n - r)) % 61u);
+      T.dig[j].w31 = (uint8_t)((lr31 * (n - r)) % 31u);
+    }
+  }
+
+  T.tw61.clear();
+  T.tw31.clear();
+  const int L = P.L;
+  // -LS caps the points of the LDS-staged centre block; it also selects the
+  // multi-pass plan when the whole residue would otherwise stay in one
+  // workgroup.
+  int c = CENTRE_LOG_C;
+  if (ls_cap) {
+    c = 0;
+    while ((2u << c) <= ls_cap && c < CENTRE_LOG_C) ++c;
+    c = std::max(c, 2);
+  }
+  if (L <= SMALL_LOG_H && !(ls_cap && L >= c + 2)) {
+    P.small = true;
+    P.outer = make_pass(0, L, T);
+    P.nseg = (uint32_t)(h / SEG_ELEMS);
+  } else {
+    P.small = false;
+    const int a = L - c;
+    int a_outer = a, a_mid = 0;
+    if (a > 9) {
+      a_mid = a / 2;
+      a_outer = a - a_mid;
+    }
+    // the outer pass needs rows of at least one carry segment
+    P.outer = make_pass(0, a_outer, T);
+    P.logS = L - a_outer;
+    P.logW = std::min(11 - a_outer, P.logS);
+    if ((1 << P.logW) < SEG_ELEMS) {
+      err = "plan error: outer pass too tall";
+      return false;
+    }
+    if (a_mid) {
+      P.has_mid = true;
+      P.mid = make_pass(a_outer, a_mid, T);
+      P.mid_logS = L - a_outer - a_mid;
+      P.mid_logW = std::min(11 - a_mid, P.mid_logS);
+    }
+    P.centre = make_pass(L - c, c, T);
+    P.nseg = (uint32_t)(h >> P.logW);
+    P.seg_elems = 1 << P.logW;
+    // The exactness argument for the carry needs 8 digits of at least 12 bits.
+    if (q < 12) {
+      err = "plan error: digit width below 12 bits on the multi-pass plan";
+      return false;
+    }
+  }
+  // centre twiddles: w_h^bitrev(j) for position 2j (the reference's w[h/2+j])
+  T.sq61.resize(h / 2);
+  T.sq31.resize(h / 2);
+  {
+    const int bits = L - 1;
+    const e61 w61 = root61(L);
+    const e31 w31 = root31(L);
+    std::vector<e61> p61(h / 2);
+    std::vector<e31> p31(h / 2);
+    e61 c61{1, 0};
+    e31 c31{1, 0};
+    for (uint64_t k = 0; k < h / 2; ++k) {
+      p61[k] = c61;
+      p31[k] = c31;
+      c61 = mul(c61, w61);
+      c31 = mul(c31, w31);
+    }
+    for (uint64_t j = 0; j < h / 2; ++j) {
+      const uint32_t br = bits > 0 ? bitrev((uint32_t)j, bits) : 0u;
+      T.sq61[j] = p61[br];
+      T.sq31[j] = p31[br];
+    }
+  }
+  return true;
+}
+
+std::string describe_plan(const DevPlan& P) {
+  std::ostringstream o;
+  auto pass = [&](const char* name, const Pass& ps) {
+    o << name << ": levels " << ps.ell0 << ".." << (ps.ell0 + ps.A - 1) << " radix";
+    for (int i = 0; i < ps.nsteps; ++i) o << (i ? "," : " ") << ps.steps[i].radix;
+    o << "\n";
+  };
+  o << "plan: h=" << P.h << " (n=" << 2ull * P.h << "), " << (P.small ? "single-workgroup LDS-resident" : "multi-pass")
+    << "\n";
+  if (P.small) {
+    pass("  k_small", P.outer);
+  } else {
+    pass("  k_outer_fwd/k_outer_inv", P.outer);
+    o << "    columns S=" << (1u << P.logS) << ", W=" << (1u << P.logW) << " per workgroup, "
+      << ((1u << P.logS) >> P.logW) << " workgroups, carry segments=" << P.nseg << "\n";
+    if (P.has_mid) {
+      pass("  k_mid<fwd/inv>", P.mid);
+      o << "    columns S=" << (1u << P.mid_logS) << ", W=" << (1u << P.mid_logW) << "\n";
+    }
+    pass("  k_centre", P.centre);
+    o << "    " << ((P.h >> P.centre.A) / 2 + 1) << " workgroups of 2 blocks x " << (1u << P.centre.A) << "\n";
+  }
+  return o.str();
+}
+
+namespace {
+
+// Non-negative big integer as 64-bit words.
+struct Big {
+  std::vector<uint64_t> w;
+  explicit Big(size_t nw) : w(nw, 0) {}
+  void add_at(uint64_t bit, unsigned __int128 v) {
+    size_t i = bit / 64;
+    const unsigned sh = bit % 64;
+    uint64_t parts[3] = {(uint64_t)(v << sh), 0, 0};
+    if (sh) {
+      parts[1] = (uint64_t)(v >> (64 - sh));
+      parts[2] = (uint64_t)((v >> 64) >> (64 - sh));
+    } else {
+      parts[1] = (uint64_t)(v >> 64);
+    }
+    unsigned __int128 c = 0;
+    for (int k = 0; k < 3 || c; ++k, ++i) {
+      if (i >= w.size()) w.push_back(0);
+      const unsigned __int128 s = (unsigned __int128)w[i] + (k < 3 ? parts[k] : 0) + c;
+      w[i] = (uint64_t)s;
+      c = s >> 64;
+      if (k >= 2 && !c) break;
+    }
+  }
+  // value mod 2^p - 1, canonical (0 for M_p)
+  void mod_mersenne(uint32_t p) {
+    const size_t pw = p / 64;
+    const unsigned pb = p % 64;
+    for (;;) {
+      bool high = false;
+      for (size_t i = pw; i < w.size(); ++i) {
+        const uint64_t x = (i == pw) ? (pb ? w[i] >> pb : w[i]) : w[i];
+        if (x) {
+          high = true;
+          break;
+        }
+      }
+      if (!high) break;
+      // hi = value >> p ; value &= 2^p - 1 ; value += hi
+      std::vector<uint64_t> hi(w.size() - pw + 1, 0);
+      for (size_t i = 0; pw + i < w.size(); ++i) {
+        const uint64_t lo = pb ? (w[pw + i] >> pb) : w[pw + i];
+        const uint64_t up = (pb && pw + i + 1 < w.size()) ? (w[pw + i + 1] << (64 - pb)) : 0;
+        hi[i] = lo | up;
+      }
+      if (pb)
+        w[pw] &= (uint64_t(1) << pb) - 1;
+      else if (pw < w.size())
+        w[pw] = 0;
+      for (size_t i = pw + 1; i < w.size(); ++i) w[i] = 0;
+      unsigned __int128 c = 0;
+      for (size_t i = 0; i < w.size(); ++i) {
+        const unsigned __int128 s = (unsigned __int128)w[i] + (i < hi.size() ? hi[i] : 0) + c;
+        w[i] = (uint64_t)s;
+        c = s >> 64;
+      }
+      if (c) w.push_back((uint64_t)c);
+    }
+    // all ones below 2^p -> 0
+    bool ones = true;
+    for (size_t i = 0; i < pw && ones; ++i) ones = (w[i] == ~0ull);
+    if (ones && pb) ones = ((w[pw] & ((uint64_t(1) << pb) - 1)) == ((uint64_t(1) << pb) - 1));
+    if (ones) std::fill(w.begin(), w.end(), 0);
+  }
+  bool ge(const Big& o) const {
+    const size_t m = std::max(w.size(), o.w.size());
+    for (size_t k = m; k-- > 0;) {
+      const uint64_t a = k < w.size() ? w[k] : 0, b = k < o.w.size() ? o.w[k] : 0;
+      if (a != b) return a > b;
+    }
+    return true;
+  }
+  void sub(const Big& o) {  // requires *this >= o
+    unsigned __int128 borrow = 0;
+    for (size_t i = 0; i < w.size(); ++i) {
+      const unsigned __int128 b = (unsigned __int128)(i < o.w.size() ? o.w[i] : 0) + borrow;
+      borrow = (unsigned __int128)w[i] < b ? 1 : 0;
+      w[i] = (uint64_t)((unsigned __int128)w[i] - b);
+    }
+  }
+};
+
+}  // namespace
+
+std::vector<uint8_t> residue_from_state(uint32_t p, const std::vector<uint32_t>& bitpos, const int64_t* digits,
+                                        uint64_t ndigits, const Carry128* carries, const uint64_t* carry_digit,
+                                        uint64_t ncarries) {
+  const size_t nw = p / 64 + 4;
+  Big pos(nw), neg(nw);
+  for (uint64_t j = 0; j < ndigits; ++j) {
+    const int64_t d = digits[j];
+    if (d > 0) pos.add_at(bitpos[j], (unsigned __int128)(uint64_t)d);
+    if (d < 0) neg.add_at(bitpos[j], (unsigned __int128)(uint64_t)(-(d + 1)) + 1u);
+  }
+  for (uint64_t s = 0; s < ncarries; ++s) {
+    const __int128 v = (__int128)(((unsigned __int128)(uint64_t)carries[s].hi << 64) | carries[s].lo);
+    if (v == 0) continue;
+    const uint64_t b = bitpos[carry_digit[s]];
+    if (v > 0) pos.add_at(b, (unsigned __int128)v);
+    else neg.add_at(b, (unsigned __int128)(-(v + 1)) + 1u);
+  }
+  pos.mod_mersenne(p);
+  neg.mod_mersenne(p);
+  if (!pos.ge(neg)) {  // pos += M_p
+    Big m(nw);
+    for (uint32_t b = 0; b < p; ++b) m.w[b / 64] |= uint64_t(1) << (b % 64);
+    Big t = pos;
+    // t = pos + M_p
+    unsigned __int128 c = 0;
+    t.w.resize(std::max(t.w.size(), m.w.size()) + 1, 0);
+    for (size_t i = 0; i < t.w.size(); ++i) {
+      const unsigned __int128 s = (unsigned __int128)t.w[i] + (i < m.w.size() ? m.w[i] : 0) + c;
+      t.w[i] = (uint64_t)s;
+      c = s >> 64;
+    }
+    pos = t;
+  }
+  pos.sub(neg);
+  pos.mod_mersenne(p);
+  const size_t nbytes = (p + 7) / 8;
+  std::vector<uint8_t> out(nbytes, 0);
+  for (size_t i = 0; i < nbytes; ++i) out[i] = (uint8_t)(pos.w[i / 8] >> (8 * (i % 8)));
+  return out;
+}
+
+void digits_from_residue(uint32_t p, const std::vector<uint32_t>& bitpos, const std::vector<DigitInfo>& dig,
+                         const uint8_t* r, size_t len, int64_t* digits) {
+  (void)p;
+  auto bit = [&](uint64_t b) -> uint64_t { return (b / 8 < len) ? ((r[b / 8] >> (b % 8)) & 1u) : 0u; };
+  for (size_t j = 0; j < dig.size(); ++j) {
+    uint64_t v = 0;
+    for (unsigned k = 0; k < dig[j].width; ++k) v |= bit((uint64_t)bitpos[j] + k) << k;
+    digits[j] = (int64_t)v;
+  }
+}
+
+}  // namespace m2

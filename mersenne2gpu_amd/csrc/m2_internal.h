@@ -1,0 +1,72 @@
+// Internal declarations shared by the engine (host) and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "m2_arith.h"
+#include "m2_plan.h"
+
+namespace m2 {
+
+// Device-side view of one test: buffers and the pass plan, passed by value
+// to the launchers.
+struct DevPlan {
+  e61* z61 = nullptr;  // h elements: spectrum (M61 part) / digits as int64 pairs
+  e31* z31 = nullptr;  // h elements: spectrum (M31 part)
+  const DigitInfo* dig = nullptr;  // n digits
+  const e61* tw61 = nullptr;       // step twiddle tables
+  const e31* tw31 = nullptr;
+  const e61* sq61 = nullptr;  // centre twiddles, h/2 entries
+  const e31* sq31 = nullptr;
+  Carry128* G = nullptr;  // pending carries, one per outer-pass row segment
+  uint32_t h = 0;
+  int ln = 0, L = 0, mode = 1;
+  bool small = false;
+  Pass outer{};  // small plan: the whole transform
+  int logS = 0, logW = 0;
+  bool has_mid = false;
+  Pass mid{};
+  int mid_logS = 0, mid_logW = 0;
+  Pass centre{};
+  uint32_t nseg = 0;  // number of carry segments (length of G)
+  int seg_elems = 0;  // elements per carry segment of the outer pass (= W)
+};
+
+hipError_t launch_small(const DevPlan& P, uint32_t iters, hipStream_t st);
+hipError_t launch_iteration(const DevPlan& P, hipStream_t st);
+// Kernel-by-kernel launch of one iteration with an event after each kernel
+// (events[0] must be recorded by the caller before); returns kernel names.
+hipError_t launch_iteration_events(const DevPlan& P, hipStream_t st, hipEvent_t* events, int max_events, int* count,
+                                   const char** names);
+
+// ---- host side (m2_host.cpp) ----
+int transform_log2(uint64_t p);
+
+struct HostTables {
+  std::vector<DigitInfo> dig;      // n
+  std::vector<uint32_t> bitpos;    // n + 1: ceil(p j / n)
+  std::vector<e61> tw61;
+  std::vector<e31> tw31;
+  std::vector<e61> sq61;  // h/2
+  std::vector<e31> sq31;
+};
+
+// Builds the pass plan (filling step twiddle offsets) and all tables.
+// ls_cap (the -LS option): 0 = automatic; otherwise the centre block is the
+// largest power of two <= ls_cap (at least 4) and the multi-pass plan is used.
+bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T, std::string& err);
+std::string describe_plan(const DevPlan& P);
+
+// Canonical residue (little-endian, ceil(p/8) bytes) of the state given by
+// signed digits plus carries pending at given digit indices.
+std::vector<uint8_t> residue_from_state(uint32_t p, const std::vector<uint32_t>& bitpos, const int64_t* digits,
+                                        uint64_t ndigits, const Carry128* carries, const uint64_t* carry_digit,
+                                        uint64_t ncarries);
+// Digits (int64) of a canonical residue.
+void digits_from_residue(uint32_t p, const std::vector<uint32_t>& bitpos, const std::vector<DigitInfo>& dig,
+                         const uint8_t* r, size_t len, int64_t* digits);
+
+}  // namespace m2

@@ -1,0 +1,52 @@
+// Transform plan shared by the host engine and the HIP kernels.
+//
+// One squaring of x mod 2^p - 1 runs on h = n/2 elements (digit pairs packed as
+// a + b i) in GF(M61^2) and GF(M31^2). The length-h transform is a sequence of
+// radix-2 levels l = 0..L-1 (L = log2 h); level l splits each block j of size
+// h/2^l with the twiddle tau = zeta_{l+1,2j} (the reference's w[2^l + j],
+// mersenne2_gpu.cpp:3911-3917). Levels are grouped into radix-8 steps (one
+// radix-2 or radix-4 step takes the remainder) and the steps into passes:
+//   * small plan (h <= SMALL_H): one workgroup keeps the whole residue in LDS
+//     and runs many iterations per launch;
+//   * large plan: an outer pass over the first A levels (columns of stride
+//     S = h/2^A, carry-fused at both ends), an optional middle pass, and a
+//     centre pass over contiguous blocks of C = 2^c elements that does the
+//     last forward levels, the real-data squaring and the first inverse levels.
+#pragma once
+#include <stdint.h>
+
+namespace m2 {
+
+constexpr int MAX_STEPS = 8;
+constexpr int SMALL_LOG_H = 11;     // h <= 2048 elements (48 KB of LDS)
+constexpr int CENTRE_LOG_C = 10;    // centre block: 1024 elements, two per workgroup
+constexpr int PASS_MAX_ELEMS = 2048;  // R * W elements per strided-pass workgroup
+constexpr int SEG_ELEMS = 4;        // carry segment: 4 elements = 8 digits
+
+// One radix step inside a pass: global level `ell`, radix 2/4/8, and the
+// offset of its twiddle block (tau^1..tau^(r-1) for each block j at level ell).
+struct Step {
+  int32_t ell;
+  int32_t radix;
+  uint32_t twoff;
+};
+
+struct Pass {
+  int32_t ell0;   // first level of the pass
+  int32_t A;      // number of levels (rows R = 2^A)
+  int32_t nsteps;
+  Step steps[MAX_STEPS];
+};
+
+// Digit metadata, one per digit (n of them): width in bits and the weight
+// shifts in each field.
+struct DigitInfo {
+  uint8_t width, w61, w31, pad;
+};
+
+struct Carry128 {
+  uint64_t lo;
+  int64_t hi;
+};
+
+}  // namespace m2

@@ -1,0 +1,143 @@
+"""Parity of the HIP engine (libm2.so through its C ABI) with exact residues.
+
+Expected values are the committed fixtures under tests/golden (exact GMP /
+Python arithmetic, cross-checked against the oracle by test_oracle.py) and,
+for randomised cases, the oracle itself. No tolerance: Res64 and residues must
+be bit-identical.
+"""
+import hashlib
+import os
+import subprocess
+
+import pytest
+
+import mersenne2gpu_amd as m2
+from conftest import load_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(ctx, case, ls=0, max_k=None, graph_iters=0):
+    ks = sorted(int(k) for k in case["checkpoints"])
+    if max_k is not None:
+        ks = [k for k in ks if k <= max_k]
+    print("case p=%d %s ls=%d ks=%s" % (case["p"], case["mode"], ls, ks), flush=True)
+    t = m2.Test(ctx, case["p"], case["mode"], graph_iters=graph_iters) if not ls else _test_ls(ctx, case, ls)
+    with t:
+        done = 0
+        for k in ks:
+            t.run(k - done)
+            done = k
+            want = case["checkpoints"][str(k)]
+            assert t.res64() == want["res64"], (case["p"], case["mode"], k, ls)
+            if "sha256" in want:
+                r = t.residue()
+                assert hashlib.sha256(r).hexdigest() == want["sha256"], (case["p"], k)
+            if "residue_hex" in want:
+                assert t.residue()[::-1].hex().upper() == want["residue_hex"]
+
+
+def _test_ls(ctx, case, ls):
+    import ctypes
+    opts = m2._Opts()
+    opts.local_stage_cap = ls
+    h = ctypes.c_void_p()
+    rc = m2.lib().m2_test_create(ctx.h, case["p"], 1 if case["mode"] == "prp" else 0, ctypes.byref(opts),
+                                 ctypes.byref(h))
+    m2._chk(rc, "m2_test_create")
+    t = m2.Test.__new__(m2.Test)
+    t.p, t.mode, t.h = case["p"], case["mode"], h
+    return t
+
+
+def test_kat_full_tests(gpu_ctx):
+    for c in load_fixture("kat.json")["cases"]:
+        _check(gpu_ctx, c)
+
+
+def test_sweep_every_transform_length(gpu_ctx):
+    for c in load_fixture("sweep.json")["cases"]:
+        _check(gpu_ctx, c)
+
+
+@pytest.mark.parametrize("ls", [4, 16, 64, 256])
+def test_sweep_multipass_plan_at_small_sizes(gpu_ctx, ls):
+    """-LS forces the multi-pass kernels (outer/centre, and the middle pass
+    when the outer levels exceed 9) onto small transforms."""
+    for c in load_fixture("sweep.json")["cases"]:
+        if 4 <= c["ln"] <= 17 and (c["p"] >> c["ln"]) >= 12:
+            _check(gpu_ctx, c, ls=ls)
+
+
+def test_c1_full_prp_tests(gpu_ctx):
+    for c in load_fixture("c1.json")["cases"]:
+        _check(gpu_ctx, c)
+
+
+def test_c1_on_multipass_plan(gpu_ctx):
+    for c in load_fixture("c1.json")["cases"][:2]:
+        _check(gpu_ctx, c, ls=64, max_k=2000)
+
+
+def test_c2_checkpoints_n2p20(gpu_ctx):
+    for c in load_fixture("c2.json")["cases"]:
+        _check(gpu_ctx, c)
+
+
+def test_c2_graph_and_direct_launches_agree(gpu_ctx):
+    c = load_fixture("c2.json")["cases"][0]
+    _check(gpu_ctx, c, max_k=100, graph_iters=1)
+    _check(gpu_ctx, c, max_k=100, graph_iters=7)
+
+
+def test_c3_ll_checkpoints_n2p24(gpu_ctx):
+    for c in load_fixture("c3.json")["cases"]:
+        _check(gpu_ctx, c)
+
+
+@pytest.mark.parametrize("p,mode,k1,k2", [(19937, "prp", 700, 1300), (27899687, "prp", 3, 10),
+                                          (132049, "ll", 50, 120)])
+def test_resume_from_residue(gpu_ctx, p, mode, k1, k2):
+    with m2.Test(gpu_ctx, p, mode) as a:
+        a.run(k1)
+        r = a.residue()
+        a.run(k2 - k1)
+        want = a.residue()
+    with m2.Test(gpu_ctx, p, mode) as b:
+        b.set_residue(r, k1)
+        assert b.iteration == k1
+        b.run(k2 - k1)
+        assert b.residue() == want
+
+
+@pytest.mark.parametrize("p,mode,prime", [(9941, "prp", True), (9941, "ll", True), (11213, "prp", True),
+                                          (10501, "prp", False), (21701, "ll", True), (21011, "ll", False)])
+def test_verdicts(gpu_ctx, p, mode, prime):
+    with m2.Test(gpu_ctx, p, mode) as t:
+        t.run(t.total_iters)
+        assert t.is_prime() == prime
+
+
+def test_cli_full_prp_test_and_residue_lines(tmp_path):
+    dump = tmp_path / "r.bin"
+    r = subprocess.run([m2.CLI_PATH, "19937", "prp", "--res64-at", "100,2000", "--dump-residue", str(dump)],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0] == "p=19937, ln=9, transform=256 (n=512)"
+    assert lines[1].startswith("device=0 (")
+    assert lines[2].startswith("carry_blocks=")
+    assert "Res64@100: EB67D29C64046DB3" in lines
+    assert "Res64@2000: B5DBC55987E2FD16" in lines
+    assert any(l.startswith("2^19937 - 1 is a probable prime, elapsed ") for l in lines)
+    assert lines[-1] == "Res64: 0000000000000009"
+    assert dump.read_bytes() == (9).to_bytes((19937 + 7) // 8, "little")
+
+
+def test_cli_ll_default_mode_and_benchmark_iters():
+    r = subprocess.run([m2.CLI_PATH, "20983", "-N", "1000"], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "benchmark_iters=1000" in r.stdout
+    assert "iter 1000/1000 (100.0%)" in r.stdout
+    assert "2^20983 - 1 is composite" in r.stdout
