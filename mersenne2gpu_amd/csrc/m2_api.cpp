@@ -112,9 +112,10 @@ int m2_test_footprint(uint64_t p, uint64_t* bytes) {
   if (!bytes) return fail(M2_E_INVAL, "null pointer");
   if (p < 3 || p > 0xFFFFFFFFull) return fail(M2_E_RANGE, "exponent out of range");
   const uint64_t n = uint64_t(1) << transform_log2(p), h = n / 2;
-  // spectrum 24 B/element, digit table 4 B/digit, twiddles <= ~1.2 h x 24 B,
-  // centre twiddles h/2 x 24 B, carries 16 B per segment
-  *bytes = 24 * h + 4 * n + 29 * h + 12 * h + 4 * h + 4096;
+  // spectrum 24 B/element, step twiddles < h x 24 B, centre twiddles
+  // h/2 x 24 B, pending carries <= 16 B per 4 elements
+  *bytes = 24 * h + 24 * h + 12 * h + 4 * h + 4096;
+  (void)n;
   return M2_OK;
 }
 
@@ -145,9 +146,16 @@ int m2_test_destroy(m2_test* t) {
   return M2_OK;
 }
 
+// digits: n digits in natural order
 static int upload_state(m2_test* t, const int64_t* digits) {
   const uint64_t h = t->P.h;
-  HIPCHK(hipMemcpyAsync(t->P.z61, digits, h * sizeof(e61), hipMemcpyHostToDevice, t->stream));
+  std::vector<int64_t> slots(2 * h);
+  for (uint64_t e = 0; e < h; ++e) {
+    const uint64_t s = digit_slot(t->P, e);
+    slots[2 * s] = digits[2 * e];
+    slots[2 * s + 1] = digits[2 * e + 1];
+  }
+  HIPCHK(hipMemcpyAsync(t->P.z61, slots.data(), h * sizeof(e61), hipMemcpyHostToDevice, t->stream));
   if (!t->P.small) {
     std::vector<Carry128> zero(t->P.nseg, Carry128{0, 0});
     HIPCHK(hipMemcpyAsync(t->P.G, zero.data(), zero.size() * sizeof(Carry128), hipMemcpyHostToDevice, t->stream));
@@ -177,8 +185,6 @@ int m2_test_create(m2_ctx* ctx, uint64_t p, int mode, const m2_opts* opts, m2_te
   t->P.z61 = (e61*)d;
   if ((rc = alloc_copy(t.get(), &d, nullptr, h * sizeof(e31)))) return rc;
   t->P.z31 = (e31*)d;
-  if ((rc = alloc_copy(t.get(), &d, t->T.dig.data(), n * sizeof(DigitInfo)))) return rc;
-  t->P.dig = (const DigitInfo*)d;
   if ((rc = alloc_copy(t.get(), &d, t->T.tw61.data(), t->T.tw61.size() * sizeof(e61)))) return rc;
   t->P.tw61 = (const e61*)d;
   if ((rc = alloc_copy(t.get(), &d, t->T.tw31.data(), t->T.tw31.size() * sizeof(e31)))) return rc;
@@ -271,16 +277,20 @@ static int read_residue(m2_test* t, std::vector<uint8_t>& r) {
   HIPCHK(hipSetDevice(t->ctx->device));
   HIPCHK(hipStreamSynchronize(t->stream));
   const uint64_t h = t->P.h, n = 2 * h;
-  std::vector<int64_t> digits(n);
-  HIPCHK(hipMemcpy(digits.data(), t->P.z61, h * sizeof(e61), hipMemcpyDeviceToHost));
+  std::vector<int64_t> slots(n), digits(n);
+  HIPCHK(hipMemcpy(slots.data(), t->P.z61, h * sizeof(e61), hipMemcpyDeviceToHost));
+  for (uint64_t e = 0; e < h; ++e) {
+    const uint64_t s = digit_slot(t->P, e);
+    digits[2 * e] = slots[2 * s];
+    digits[2 * e + 1] = slots[2 * s + 1];
+  }
   std::vector<Carry128> G;
   std::vector<uint64_t> at;
   if (!t->P.small) {
     G.resize(t->P.nseg);
     HIPCHK(hipMemcpy(G.data(), t->P.G, G.size() * sizeof(Carry128), hipMemcpyDeviceToHost));
     at.resize(G.size());
-    // segment s covers elements [s W, s W + W); its carry enters the next one
-    for (uint64_t s = 0; s < G.size(); ++s) at[s] = 2ull * (((s + 1) % G.size()) * (uint64_t)t->P.seg_elems);
+    for (uint64_t g = 0; g < G.size(); ++g) at[g] = carry_target_digit(t->P, g);
   }
   r = residue_from_state(t->p, t->T.bitpos, digits.data(), n, G.data(), at.data(), G.size());
   return M2_OK;

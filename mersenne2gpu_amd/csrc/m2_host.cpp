@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <sstream>
 
+#include "m2_digits.h"
 #include "m2_internal.h"
 
 namespace m2 {
@@ -106,6 +107,10 @@ bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T
     err = "exponent too large: transform length above 2^27";
     return false;
   }
+  if ((p & 1u) == 0) {
+    err = "exponent must be odd";
+    return false;
+  }
   const uint64_t n = uint64_t(1) << ln, h = n / 2;
   const uint32_t q = p >> ln;
   P.ln = ln;
@@ -132,6 +137,38 @@ bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T
       const uint64_t r = ((uint64_t)p * j) & (n - 1);
       T.dig[j].w61 = (uint8_t)((lr61 * (n - r)) % 61u);
       T.dig[j].w31 = (uint8_t)((lr31 * (n - r)) % 31u);
+    }
+  }
+
+  DigitGen& D = P.dgen;
+  D.p = p;
+  D.nmask = (uint32_t)(n - 1);
+  D.s = (uint32_t)(p & (n - 1));
+  D.q = q;
+  D.lr61 = (uint32_t)lr61;
+  D.lr31 = (uint32_t)lr31;
+  D.d61 = (uint32_t)((lr61 * D.s) % 61u);
+  D.d31 = (uint32_t)((lr31 * D.s) % 31u);
+  D.k61 = (uint32_t)((ln + 1) % 61);
+  D.k31 = (uint32_t)((ln + 1) % 31);
+
+  // The kernels walk widths and weights incrementally; check the walk against
+  // the tables above for every digit.
+  {
+    DWalk w = dwalk_at(D, 0);
+    for (uint64_t j = 0; j < n; ++j) {
+      if (j % 4096 == 0) {
+        const DWalk f = dwalk_at(D, (uint32_t)j);
+        if (f.r != w.r || f.v61 != w.v61 || f.v31 != w.v31) {
+          err = "digit walk: restart mismatch";
+          return false;
+        }
+      }
+      if (dwalk_width(D, w) != T.dig[j].width || dwalk_w61(w) != T.dig[j].w61 || dwalk_w31(w) != T.dig[j].w31) {
+        err = "digit walk mismatch at digit " + std::to_string(j);
+        return false;
+      }
+      dwalk_next(D, w);
     }
   }
 

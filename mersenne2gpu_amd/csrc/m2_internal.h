@@ -16,7 +16,6 @@ namespace m2 {
 struct DevPlan {
   e61* z61 = nullptr;  // h elements: spectrum (M61 part) / digits as int64 pairs
   e31* z31 = nullptr;  // h elements: spectrum (M31 part)
-  const DigitInfo* dig = nullptr;  // n digits
   const e61* tw61 = nullptr;       // step twiddle tables
   const e31* tw31 = nullptr;
   const e61* sq61 = nullptr;  // centre twiddles, h/2 entries
@@ -33,7 +32,28 @@ struct DevPlan {
   Pass centre{};
   uint32_t nseg = 0;  // number of carry segments (length of G)
   int seg_elems = 0;  // elements per carry segment of the outer pass (= W)
+  DigitGen dgen{};
 };
+
+// Position of element e's digit pair in z61 between iterations: natural order
+// on the small plan, strip-major (strip cg = column / W, then row, then
+// column within the strip) on the multi-pass plan.
+inline uint64_t digit_slot(const DevPlan& P, uint64_t e) {
+  if (P.small) return e;
+  const uint64_t S = uint64_t(1) << P.logS, W = uint64_t(1) << P.logW, R = uint64_t(1) << P.outer.A;
+  const uint64_t r = e >> P.logS, c = e & (S - 1);
+  return ((c >> P.logW) * R + r) * W + (c & (W - 1));
+}
+// First digit of the strip-row that carry G[g] enters (g = strip * R + row).
+inline uint64_t carry_target_digit(const DevPlan& P, uint64_t g) {
+  const uint64_t R = uint64_t(1) << P.outer.A, NS = (uint64_t(1) << P.logS) >> P.logW;
+  uint64_t cg = g / R, r = g % R;
+  if (++cg == NS) {
+    cg = 0;
+    if (++r == R) r = 0;
+  }
+  return 2 * ((r << P.logS) + (cg << P.logW));
+}
 
 hipError_t launch_small(const DevPlan& P, uint32_t iters, hipStream_t st);
 hipError_t launch_iteration(const DevPlan& P, hipStream_t st);

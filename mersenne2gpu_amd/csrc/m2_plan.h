@@ -18,7 +18,7 @@
 namespace m2 {
 
 constexpr int MAX_STEPS = 8;
-constexpr int SMALL_LOG_H = 11;     // h <= 2048 elements (48 KB of LDS)
+constexpr int SMALL_LOG_H = 11;     // h <= 2048 elements (48 KB of data in LDS)
 constexpr int CENTRE_LOG_C = 10;    // centre block: 1024 elements, two per workgroup
 constexpr int PASS_MAX_ELEMS = 2048;  // R * W elements per strided-pass workgroup
 constexpr int SEG_ELEMS = 4;        // carry segment: 4 elements = 8 digits
@@ -39,9 +39,26 @@ struct Pass {
 };
 
 // Digit metadata, one per digit (n of them): width in bits and the weight
-// shifts in each field.
+// shifts in each field (host tables; the kernels derive the same values on
+// the fly from DigitGen).
 struct DigitInfo {
   uint8_t width, w61, w31, pad;
+};
+
+// Constants from which a kernel walks the digit metadata incrementally. With
+// r_j = p j mod n (p odd) and s = p mod n:
+//   width_j = q + [r_j + s > n] + [r_j = 0]
+//   w61_j   = lr61 (n - r_j) mod 61 for r_j != 0, else 0   (same for 31)
+// and from digit j to j+1: r += s (mod n), w61' = w61' - d61 + [wrap] (mod 61),
+// where w61' = lr61 (n - r) mod 61 is tracked for every r (it is 1 at r = 0).
+struct DigitGen {
+  uint32_t p;
+  uint32_t nmask;  // n - 1
+  uint32_t s;      // p mod n
+  uint32_t q;      // p >> ln
+  uint32_t lr61, lr31;
+  uint32_t d61, d31;  // lr * s mod 61 / 31
+  uint32_t k61, k31;  // (ln + 1) mod 61 / 31: extra right shift of the unweight
 };
 
 struct Carry128 {

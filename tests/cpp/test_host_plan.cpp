@@ -1,0 +1,82 @@
+// CPU checks of the engine's host code (no GPU calls): plan construction and
+// its digit-walk self-check, digit/carry layouts, and exact residue packing.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <random>
+#include <set>
+#include <vector>
+
+#include "m2_internal.h"
+
+using namespace m2;
+
+static int fails = 0;
+#define CHECK(c, ...)                  \
+  do {                                 \
+    if (!(c)) {                        \
+      printf("FAIL %s: ", #c);         \
+      printf(__VA_ARGS__);             \
+      printf("\n");                    \
+      ++fails;                         \
+    }                                  \
+  } while (0)
+
+int main() {
+  std::mt19937_64 rng(42);
+  const uint32_t ps[] = {3, 5, 7, 127, 349, 353, 607, 1279, 19937, 20983, 21001, 132049, 2424827, 27899687};
+  const uint32_t caps[] = {0, 4, 16, 64, 256};
+  for (uint32_t p : ps) {
+    for (uint32_t cap : caps) {
+      DevPlan P;
+      HostTables T;
+      std::string err;
+      const bool ok = build_plan(p, 1, cap, P, T, err);
+      if (!ok) {
+        // only the multi-pass carry bound (q >= 12) may refuse a plan
+        CHECK(err.find("12 bits") != std::string::npos, "p=%u cap=%u: %s", p, cap, err.c_str());
+        continue;
+      }
+      const uint64_t h = P.h, n = 2 * h;
+      // layout is a permutation
+      std::vector<char> seen(h, 0);
+      for (uint64_t e = 0; e < h; ++e) {
+        const uint64_t s = digit_slot(P, e);
+        CHECK(s < h && !seen[s], "p=%u cap=%u slot %llu", p, cap, (unsigned long long)s);
+        if (s < h) seen[s] = 1;
+      }
+      if (!P.small) {
+        std::set<uint64_t> tgt;
+        for (uint64_t g = 0; g < P.nseg; ++g) tgt.insert(carry_target_digit(P, g));
+        CHECK(tgt.size() == P.nseg, "p=%u carry targets not distinct", p);
+        CHECK(tgt.count(0) == 1, "p=%u no carry wraps to digit 0", p);
+      }
+      // residue round trip on random digits, including negatives and carries
+      if (cap == 0 && n <= (1u << 16)) {
+        std::vector<int64_t> d(n);
+        for (uint64_t j = 0; j < n; ++j) d[j] = (int64_t)(rng() % (1ull << T.dig[j].width));
+        std::vector<uint8_t> r = residue_from_state(p, T.bitpos, d.data(), n, nullptr, nullptr, 0);
+        std::vector<int64_t> d2(n);
+        digits_from_residue(p, T.bitpos, T.dig, r.data(), r.size(), d2.data());
+        std::vector<uint8_t> r2 = residue_from_state(p, T.bitpos, d2.data(), n, nullptr, nullptr, 0);
+        CHECK(r == r2, "p=%u residue round trip", p);
+        // moving one unit between digit j and the next is value-neutral:
+        // d_j += 2^w_j, d_{j+1} -= 1  (here on j = 1, so digit 2 may go negative)
+        std::vector<int64_t> d3 = d2;
+        d3[1] += (int64_t)1 << T.dig[1].width;
+        d3[2] -= 1;
+        CHECK(residue_from_state(p, T.bitpos, d3.data(), n, nullptr, nullptr, 0) == r, "p=%u carry move", p);
+        // a pending carry at digit 0 of 1 equals adding 1 to digit 0
+        Carry128 c{1, 0};
+        uint64_t at = 0;
+        std::vector<int64_t> d4 = d2;
+        d4[0] += 1;
+        CHECK(residue_from_state(p, T.bitpos, d2.data(), n, &c, &at, 1) ==
+                  residue_from_state(p, T.bitpos, d4.data(), n, nullptr, nullptr, 0),
+              "p=%u pending carry", p);
+      }
+    }
+  }
+  printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
+  return fails ? 1 : 0;
+}

@@ -1,0 +1,16 @@
+"""Builds and runs the C++ host-code test (tests/cpp/test_host_plan.cpp) on the CPU."""
+import os
+import subprocess
+
+from conftest import ROOT
+
+
+def test_host_plan_and_residue_code(tmp_path):
+    csrc = os.path.join(ROOT, "mersenne2gpu_amd", "csrc")
+    exe = str(tmp_path / "test_host_plan")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-O1", "-std=c++17",
+                    "-I", csrc, "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "test_host_plan.cpp"), os.path.join(csrc, "m2_host.cpp"),
+                    "-o", exe], check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:]
