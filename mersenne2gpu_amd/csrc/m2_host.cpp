@@ -265,7 +265,7 @@ std::string describe_plan(const DevPlan& P) {
       o << "    columns S=" << (1u << P.mid_logS) << ", W=" << (1u << P.mid_logW) << "\n";
     }
     pass("  k_centre", P.centre);
-    o << "    " << ((P.h >> P.centre.A) / 2 + 1) << " workgroups of 2 blocks x " << (1u << P.centre.A) << "\n";
+    o << "    " << ((P.h >> P.centre.A) / 2) << " workgroups of 2 blocks x " << (1u << P.centre.A) << "\n";
   }
   return o.str();
 }

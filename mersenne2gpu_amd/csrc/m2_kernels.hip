@@ -35,6 +35,12 @@ typedef __int128 i128;
 
 __device__ __forceinline__ int64_t to_i64(uint64_t x) { return (int64_t)x; }
 
+// LDS data index with one pad slot per 8 elements. Radix steps access
+// elements 8^k apart; the pad spreads those (and 8-element butterfly groups)
+// over distinct banks.
+__device__ __forceinline__ int PADX(int i) { return i + (i >> 3); }
+constexpr int padded(int n) { return n + n / 8; }
+
 // Signed digit -> canonical field residues.
 __device__ __forceinline__ uint64_t digit61(int64_t d) { return d >= 0 ? (uint64_t)d : P61 - (uint64_t)(-d); }
 __device__ __forceinline__ uint32_t mod31_u64(uint64_t x) {
@@ -84,8 +90,8 @@ __device__ __forceinline__ i128 seg_normalise(e61* L61, const e31* L31, int idx0
   DWalk w = dwalk_at(D, j0);
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
-    const e61 v = L61[idx0 + i];
-    const e31 y = L31[idx0 + i];
+    const e61 v = L61[PADX(idx0 + i)];
+    const e31 y = L31[PADX(idx0 + i)];
     const uint32_t wa = dwalk_width(D, w);
     i128 xa = crt_digit(v.a, y.a, dwalk_w61(w), dwalk_w31(w), D);
     dwalk_next(D, w);
@@ -94,7 +100,7 @@ __device__ __forceinline__ i128 seg_normalise(e61* L61, const e31* L31, int idx0
     dwalk_next(D, w);
     const int64_t da = norm_digit(xa + carry, wa, carry);
     const int64_t db = norm_digit(xb + carry, wb, carry);
-    L61[idx0 + i] = {(uint64_t)da, (uint64_t)db};
+    L61[PADX(idx0 + i)] = {(uint64_t)da, (uint64_t)db};
   }
   return carry;
 }
@@ -106,7 +112,7 @@ __device__ __forceinline__ void seg_ripple(e61* L61, int idx0, uint32_t j0, cons
   int64_t d[2 * SEG_ELEMS];
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
-    const e61 v = L61[idx0 + i];
+    const e61 v = L61[PADX(idx0 + i)];
     d[2 * i] = to_i64(v.a);
     d[2 * i + 1] = to_i64(v.b);
   }
@@ -120,7 +126,7 @@ __device__ __forceinline__ void seg_ripple(e61* L61, int idx0, uint32_t j0, cons
   }
   d[2 * SEG_ELEMS - 1] += (int64_t)c;
 #pragma unroll
-  for (int i = 0; i < SEG_ELEMS; ++i) L61[idx0 + i] = {(uint64_t)d[2 * i], (uint64_t)d[2 * i + 1]};
+  for (int i = 0; i < SEG_ELEMS; ++i) L61[PADX(idx0 + i)] = {(uint64_t)d[2 * i], (uint64_t)d[2 * i + 1]};
 }
 
 // Digits of a segment -> weighted field elements in place.
@@ -128,14 +134,14 @@ __device__ __forceinline__ void seg_weight(e61* L61, e31* L31, int idx0, uint32_
   DWalk w = dwalk_at(D, j0);
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
-    const e61 v = L61[idx0 + i];
+    const e61 v = L61[PADX(idx0 + i)];
     const int64_t da = to_i64(v.a), db = to_i64(v.b);
     const uint32_t a61 = dwalk_w61(w), a31 = dwalk_w31(w);
     dwalk_next(D, w);
     const uint32_t b61 = dwalk_w61(w), b31 = dwalk_w31(w);
     dwalk_next(D, w);
-    L61[idx0 + i] = {rot61(digit61(da), a61), rot61(digit61(db), b61)};
-    L31[idx0 + i] = {rot31(digit31(da), a31), rot31(digit31(db), b31)};
+    L61[PADX(idx0 + i)] = {rot61(digit61(da), a61), rot61(digit61(db), b61)};
+    L31[PADX(idx0 + i)] = {rot31(digit31(da), a31), rot31(digit31(db), b31)};
   }
 }
 
@@ -194,8 +200,8 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
     e31 x[RADIX];
 #pragma unroll
     for (int e = 0; e < RADIX; ++e) {
-      v[e] = L61[base + e * stride];
-      x[e] = L31[base + e * stride];
+      v[e] = L61[PADX(base + e * stride)];
+      x[e] = L31[PADX(base + e * stride)];
     }
     if (FWD) {
 #pragma unroll
@@ -218,8 +224,8 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
     }
 #pragma unroll
     for (int e = 0; e < RADIX; ++e) {
-      L61[base + e * stride] = v[e];
-      L31[base + e * stride] = x[e];
+      L61[PADX(base + e * stride)] = v[e];
+      L31[PADX(base + e * stride)] = x[e];
     }
   }
   __syncthreads();
@@ -260,39 +266,42 @@ __device__ __forceinline__ void square_pair(E& zk, E& zmk, E tw) {
   zmk = conj(sub(v0, v1));
 }
 
-// Squaring inside one LDS block that pairs with itself: block 0 (positions
-// 0..Cb-1, pairs via the mirror formula, k = 0 special) or block 1
-// (positions Cb..2Cb-1, k pairs with 3Cb-1-k). S61/S31: the centre twiddles
-// of positions 2j of the block (Cb/2 entries), already in LDS.
-__device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, uint32_t B, const e61* S61,
+// Squaring of blocks that pair with themselves: block 0 (positions 0..Cb-1,
+// mirror formula, k = 0 special) at LDS offset 0, and, when nself = 2, block 1
+// (positions Cb..2Cb-1, k pairs with 3Cb-1-k) at LDS offset Cb. S61/S31 hold
+// the centre twiddles of the even positions: block 0's Cb/2, then block 1's.
+__device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int nself, const e61* S61,
                                             const e31* S31) {
   const int Cb = 1 << logCb;
-  for (int j = threadIdx.x; j < (Cb >> 1); j += blockDim.x) {
+  const int half = Cb >> 1;
+  for (int q = threadIdx.x; q < nself * half; q += blockDim.x) {
+    const int blk = q >= half ? 1 : 0;
+    const int j = q - blk * half;
     const int k = 2 * j;
-    const int kidx = k;
-    const int mkidx = (B == 0) ? (k ? (3 << (31 - __clz(k))) - k - 1 : 0) : Cb - 1 - k;
-    const e61 t61 = S61[j];
-    const e31 t31 = S31[j];
-    if (B == 0 && k == 0) {
-      e61 a = L61[0], a2 = a;
-      e31 b = L31[0], b2 = b;
+    const int kidx = blk * Cb + k;
+    const int mkidx = blk ? Cb + (Cb - 1 - k) : (k ? (3 << (31 - __clz(k))) - k - 1 : 0);
+    const e61 t61 = S61[q];
+    const e31 t31 = S31[q];
+    if (blk == 0 && k == 0) {
+      e61 a = L61[PADX(0)], a2 = a;
+      e31 b = L31[PADX(0)], b2 = b;
       square_pair(a, a2, t61);
       square_pair(b, b2, t31);
-      const e61 z1 = L61[1];
-      const e31 y1 = L31[1];
-      L61[0] = a;
-      L31[0] = b;
-      L61[1] = sqr(add(z1, z1));
-      L31[1] = sqr(add(y1, y1));
+      const e61 z1 = L61[PADX(1)];
+      const e31 y1 = L31[PADX(1)];
+      L61[PADX(0)] = a;
+      L31[PADX(0)] = b;
+      L61[PADX(1)] = sqr(add(z1, z1));
+      L31[PADX(1)] = sqr(add(y1, y1));
     } else {
-      e61 a = L61[kidx], c = L61[mkidx];
-      e31 b = L31[kidx], d = L31[mkidx];
+      e61 a = L61[PADX(kidx)], c = L61[PADX(mkidx)];
+      e31 b = L31[PADX(kidx)], d = L31[PADX(mkidx)];
       square_pair(a, c, t61);
       square_pair(b, d, t31);
-      L61[kidx] = a;
-      L61[mkidx] = c;
-      L31[kidx] = b;
-      L31[mkidx] = d;
+      L61[PADX(kidx)] = a;
+      L61[PADX(mkidx)] = c;
+      L31[PADX(kidx)] = b;
+      L31[PADX(mkidx)] = d;
     }
   }
   __syncthreads();
@@ -314,14 +323,14 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, cons
       mkidx = o;
       ti = (Cb >> 1) + ((Cb - 1 - o) >> 1);
     }
-    e61 a = L61[kidx], c = L61[mkidx];
-    e31 b = L31[kidx], d = L31[mkidx];
+    e61 a = L61[PADX(kidx)], c = L61[PADX(mkidx)];
+    e31 b = L31[PADX(kidx)], d = L31[PADX(mkidx)];
     square_pair(a, c, S61[ti]);
     square_pair(b, d, S31[ti]);
-    L61[kidx] = a;
-    L61[mkidx] = c;
-    L31[kidx] = b;
-    L31[mkidx] = d;
+    L61[PADX(kidx)] = a;
+    L61[PADX(mkidx)] = c;
+    L31[PADX(kidx)] = b;
+    L31[PADX(mkidx)] = d;
   }
   __syncthreads();
 }
@@ -341,8 +350,8 @@ __global__ void __launch_bounds__(256) k_small(e61* __restrict__ z61, const e61*
                                                const e31* __restrict__ tw31, const e61* __restrict__ sq61,
                                                const e31* __restrict__ sq31, Pass ps, DigitGen D, int mode,
                                                uint32_t iters) {
-  __shared__ e61 L61[1 << SMALL_LOG_H];
-  __shared__ e31 L31[1 << SMALL_LOG_H];
+  __shared__ e61 L61[padded(1 << SMALL_LOG_H)];
+  __shared__ e31 L31[padded(1 << SMALL_LOG_H)];
   __shared__ e61 T61[1 << SMALL_LOG_H];
   __shared__ e31 T31[1 << SMALL_LOG_H];
   __shared__ e61 S61[1 << (SMALL_LOG_H - 1)];
@@ -354,13 +363,13 @@ __global__ void __launch_bounds__(256) k_small(e61* __restrict__ z61, const e61*
   const uint32_t j0[1] = {0};
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   stage_range(S61, S31, sq61, sq31, 0, h >> 1);
-  for (int i = threadIdx.x; i < h; i += blockDim.x) L61[i] = z61[i];
+  for (int i = threadIdx.x; i < h; i += blockDim.x) L61[PADX(i)] = z61[i];
   __syncthreads();
   for (uint32_t it = 0; it < iters; ++it) {
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_weight(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D);
     __syncthreads();
     lds_pass<true>(L61, L31, 0, 1, ps, T61, T31);
-    square_self(L61, L31, logH, 0, S61, S31);
+    square_self(L61, L31, logH, 1, S61, S31);
     lds_pass<false>(L61, L31, 0, 1, ps, T61, T31);
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
       const i128 c0 = (mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
@@ -371,7 +380,7 @@ __global__ void __launch_bounds__(256) k_small(e61* __restrict__ z61, const e61*
       seg_ripple(L61, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D, load_carry(gs, (s + nseg - 1) % nseg));
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < h; i += blockDim.x) z61[i] = L61[i];
+  for (int i = threadIdx.x; i < h; i += blockDim.x) z61[i] = L61[PADX(i)];
 }
 
 // ---------------------------------------------------------------------------
@@ -393,8 +402,8 @@ __global__ void __launch_bounds__(256) k_outer_fwd(e61* __restrict__ z61, e31* _
                                                    const e61* __restrict__ tw61, const e31* __restrict__ tw31,
                                                    const Carry128* __restrict__ G, Pass ps, int logS, int logW,
                                                    DigitGen D) {
-  __shared__ e61 L61[PASS_MAX_ELEMS];
-  __shared__ e31 L31[PASS_MAX_ELEMS];
+  __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
+  __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
   const int R = 1 << ps.A, W = 1 << logW;
@@ -403,7 +412,7 @@ __global__ void __launch_bounds__(256) k_outer_fwd(e61* __restrict__ z61, e31* _
   const uint32_t j0[1] = {0};
   // digits of the strip are contiguous
   const e61* src = z61 + (size_t)cg * (R * W);
-  for (int i = threadIdx.x; i < R * W; i += blockDim.x) L61[i] = src[i];
+  for (int i = threadIdx.x; i < R * W; i += blockDim.x) L61[PADX(i)] = src[i];
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   // prologue: carry-in of each strip-row, then weight
@@ -419,8 +428,8 @@ __global__ void __launch_bounds__(256) k_outer_fwd(e61* __restrict__ z61, e31* _
   lds_pass<true>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
-    z61[e] = L61[i];
-    z31[e] = L31[i];
+    z61[e] = L61[PADX(i)];
+    z31[e] = L31[PADX(i)];
   }
 }
 
@@ -428,8 +437,8 @@ __global__ void __launch_bounds__(256) k_outer_inv(e61* __restrict__ z61, const 
                                                    const e61* __restrict__ tw61, const e31* __restrict__ tw31,
                                                    Carry128* __restrict__ G, Pass ps, int logS, int logW, DigitGen D,
                                                    int mode) {
-  __shared__ e61 L61[PASS_MAX_ELEMS];
-  __shared__ e31 L31[PASS_MAX_ELEMS];
+  __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
+  __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
   __shared__ Carry128 gs[PASS_MAX_ELEMS / SEG_ELEMS];
@@ -439,8 +448,8 @@ __global__ void __launch_bounds__(256) k_outer_inv(e61* __restrict__ z61, const 
   const uint32_t j0[1] = {0};
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
-    L61[i] = z61[e];
-    L31[i] = z31[e];
+    L61[PADX(i)] = z61[e];
+    L31[PADX(i)] = z31[e];
   }
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
@@ -463,7 +472,7 @@ __global__ void __launch_bounds__(256) k_outer_inv(e61* __restrict__ z61, const 
   }
   __syncthreads();
   e61* dst = z61 + (size_t)cg * (R * W);
-  for (int i = threadIdx.x; i < R * W; i += blockDim.x) dst[i] = L61[i];
+  for (int i = threadIdx.x; i < R * W; i += blockDim.x) dst[i] = L61[PADX(i)];
 }
 
 // Middle pass: levels [ell0, ell0 + A) with no carry work. Level-ell0 block
@@ -472,8 +481,8 @@ template <bool FWD>
 __global__ void __launch_bounds__(256) k_mid(e61* __restrict__ z61, e31* __restrict__ z31,
                                              const e61* __restrict__ tw61, const e31* __restrict__ tw31, Pass ps,
                                              int logS, int logW) {
-  __shared__ e61 L61[PASS_MAX_ELEMS];
-  __shared__ e31 L31[PASS_MAX_ELEMS];
+  __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
+  __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
   const int R = 1 << ps.A, W = 1 << logW;
@@ -483,65 +492,60 @@ __global__ void __launch_bounds__(256) k_mid(e61* __restrict__ z61, e31* __restr
   const uint32_t j0[1] = {jb};
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1));
-    L61[i] = z61[e];
-    L31[i] = z31[e];
+    L61[PADX(i)] = z61[e];
+    L31[PADX(i)] = z31[e];
   }
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   lds_pass<FWD>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1));
-    z61[e] = L61[i];
-    z31[e] = L31[i];
+    z61[e] = L61[PADX(i)];
+    z31[e] = L31[PADX(i)];
   }
 }
 
 // Centre pass: workgroup g squares the pair of contiguous blocks (B, B2) of
-// C = 2^logC elements that the real-data pairing links (g = 0, 1: blocks 0
-// and 1 pair with themselves); forward levels [ps.ell0, L), square, inverse.
+// C = 2^logC elements that the real-data pairing links; workgroup 0 takes
+// blocks 0 and 1, which pair with themselves. Forward levels [ps.ell0, L),
+// square, inverse. There are h / (2C) workgroups.
 __global__ void __launch_bounds__(256) k_centre(e61* __restrict__ z61, e31* __restrict__ z31,
                                                 const e61* __restrict__ tw61, const e31* __restrict__ tw31,
                                                 const e61* __restrict__ sq61, const e31* __restrict__ sq31, Pass ps) {
-  __shared__ e61 L61[2 << CENTRE_LOG_C];
-  __shared__ e31 L31[2 << CENTRE_LOG_C];
+  __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
+  __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
   __shared__ e61 T61[2 << CENTRE_LOG_C];
   __shared__ e31 T31[2 << CENTRE_LOG_C];
   __shared__ e61 S61[1 << CENTRE_LOG_C];
   __shared__ e31 S31[1 << CENTRE_LOG_C];
   const int logC = ps.A, C = 1 << logC;
   const uint32_t g = blockIdx.x;
-  uint32_t B, B2;
-  int nb;
-  if (g < 2) {
-    B = g;
-    B2 = g;
-    nb = 1;
-  } else {
-    const uint32_t gp = g - 1;           // >= 1
-    const uint32_t a = 32u - __clz(gp);  // octave: blocks [2^a, 2^(a+1))
-    B = (1u << a) + (gp - (1u << (a - 1)));
+  uint32_t B = 0, B2 = 1;
+  const int nb = 2;
+  if (g > 0) {
+    const uint32_t a = 32u - __clz(g);  // octave: blocks [2^a, 2^(a+1))
+    B = (1u << a) + (g - (1u << (a - 1)));
     B2 = 3u * (1u << a) - 1u - B;
-    nb = 2;
   }
   const uint32_t j0[2] = {B, B2};
   for (int i = threadIdx.x; i < (nb << logC); i += blockDim.x) {
     const uint32_t e = ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1));
-    L61[i] = z61[e];
-    L31[i] = z31[e];
+    L61[PADX(i)] = z61[e];
+    L31[PADX(i)] = z31[e];
   }
   stage_pass_twiddles(T61, T31, ps, nb, j0, tw61, tw31);
   // centre twiddles of the even positions of each block
   stage_range(S61, S31, sq61, sq31, B << (logC - 1), C >> 1);
-  if (nb == 2) stage_range(S61 + (C >> 1), S31 + (C >> 1), sq61, sq31, B2 << (logC - 1), C >> 1);
+  stage_range(S61 + (C >> 1), S31 + (C >> 1), sq61, sq31, B2 << (logC - 1), C >> 1);
   __syncthreads();
   lds_pass<true>(L61, L31, 0, nb, ps, T61, T31);
-  if (nb == 1) square_self(L61, L31, logC, B, S61, S31);
+  if (g == 0) square_self(L61, L31, logC, 2, S61, S31);
   else square_cross(L61, L31, logC, S61, S31);
   lds_pass<false>(L61, L31, 0, nb, ps, T61, T31);
   for (int i = threadIdx.x; i < (nb << logC); i += blockDim.x) {
     const uint32_t e = ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1));
-    z61[e] = L61[i];
-    z31[e] = L31[i];
+    z61[e] = L61[PADX(i)];
+    z31[e] = L31[PADX(i)];
   }
 }
 
@@ -569,7 +573,7 @@ hipError_t iteration(const DevPlan& P, hipStream_t st, F&& mark) {
                        P.mid_logW);
     if ((e = mark("k_mid_fwd")) != hipSuccess) return e;
   }
-  const uint32_t ncentre = (P.h >> P.centre.A) / 2 + 1;
+  const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   hipLaunchKernelGGL(k_centre, dim3(ncentre), dim3(256), 0, st, P.z61, P.z31, P.tw61, P.tw31, P.sq61, P.sq31,
                      P.centre);
   if ((e = mark("k_centre")) != hipSuccess) return e;
