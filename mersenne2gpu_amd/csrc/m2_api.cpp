@@ -112,9 +112,9 @@ int m2_test_footprint(uint64_t p, uint64_t* bytes) {
   if (!bytes) return fail(M2_E_INVAL, "null pointer");
   if (p < 3 || p > 0xFFFFFFFFull) return fail(M2_E_RANGE, "exponent out of range");
   const uint64_t n = uint64_t(1) << transform_log2(p), h = n / 2;
-  // spectrum 24 B/element, step twiddles < h x 24 B, centre twiddles
-  // h/2 x 24 B, pending carries <= 16 B per 4 elements
-  *bytes = 24 * h + 24 * h + 12 * h + 4 * h + 4096;
+  // spectrum 24 B/element, digits 16 B/element, step twiddles < h x 24 B,
+  // centre twiddles h/2 x 24 B, pending carries <= 16 B per 4 elements
+  *bytes = 24 * h + 16 * h + 24 * h + 12 * h + 4 * h + 4096;
   (void)n;
   return M2_OK;
 }
@@ -155,7 +155,8 @@ static int upload_state(m2_test* t, const int64_t* digits) {
     slots[2 * s] = digits[2 * e];
     slots[2 * s + 1] = digits[2 * e + 1];
   }
-  HIPCHK(hipMemcpyAsync(t->P.z61, slots.data(), h * sizeof(e61), hipMemcpyHostToDevice, t->stream));
+  HIPCHK(hipMemcpyAsync(t->P.small ? t->P.z61 : t->P.zd, slots.data(), h * sizeof(e61), hipMemcpyHostToDevice,
+                        t->stream));
   if (!t->P.small) {
     std::vector<Carry128> zero(t->P.nseg, Carry128{0, 0});
     HIPCHK(hipMemcpyAsync(t->P.G, zero.data(), zero.size() * sizeof(Carry128), hipMemcpyHostToDevice, t->stream));
@@ -196,6 +197,8 @@ int m2_test_create(m2_ctx* ctx, uint64_t p, int mode, const m2_opts* opts, m2_te
   if (!t->P.small) {
     if ((rc = alloc_copy(t.get(), &d, nullptr, t->P.nseg * sizeof(Carry128)))) return rc;
     t->P.G = (Carry128*)d;
+    if ((rc = alloc_copy(t.get(), &d, nullptr, h * sizeof(e61)))) return rc;
+    t->P.zd = (e61*)d;
   }
   // the host copies of the large twiddle tables are no longer needed
   std::vector<e61>().swap(t->T.tw61);
@@ -278,7 +281,7 @@ static int read_residue(m2_test* t, std::vector<uint8_t>& r) {
   HIPCHK(hipStreamSynchronize(t->stream));
   const uint64_t h = t->P.h, n = 2 * h;
   std::vector<int64_t> slots(n), digits(n);
-  HIPCHK(hipMemcpy(slots.data(), t->P.z61, h * sizeof(e61), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(slots.data(), t->P.small ? t->P.z61 : t->P.zd, h * sizeof(e61), hipMemcpyDeviceToHost));
   for (uint64_t e = 0; e < h; ++e) {
     const uint64_t s = digit_slot(t->P, e);
     digits[2 * e] = slots[2 * s];

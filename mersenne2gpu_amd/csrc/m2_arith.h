@@ -1,10 +1,12 @@
 // Exact arithmetic in GF(M61^2) and GF(M31^2), i^2 = -1, for gfx950 HIP kernels
 // and for the host table builder (the same inline functions compile for both).
 //
-// Elements are canonical (0 <= x < M) between operations. A product is a 64x64
-// (or 32x32) multiply followed by the Mersenne fold x = lo + hi (mod 2^k - 1).
-// Multiplication by a power of two is a rotation of the k-bit word, which is
-// how the IBDWT weights are applied.
+// Elements lie in [0, M] between operations, M = 2^k - 1 standing for 0 as
+// well, so that every reduction is a Mersenne fold x = lo + hi (mod 2^k - 1)
+// or an unsigned min, without compares. A product is 32x32->64 multiplies
+// (v_mad_u64_u32) whose pieces are folded together. Multiplication by a power
+// of two is a rotation of the k-bit word, which is how the IBDWT weights are
+// applied.
 #pragma once
 #include <stdint.h>
 
@@ -24,40 +26,59 @@ struct alignas(16) e61 { uint64_t a, b; };  // a + b i in GF(M61^2)
 struct alignas(8) e31 { uint32_t a, b; };   // a + b i in GF(M31^2)
 
 // ---- M61 ----
-M2_HD uint64_t add61(uint64_t x, uint64_t y) { uint64_t t = x + y; return t >= P61 ? t - P61 : t; }
-M2_HD uint64_t sub61(uint64_t x, uint64_t y) { return x >= y ? x - y : x + P61 - y; }
-M2_HD uint64_t neg61(uint64_t x) { return x ? P61 - x : 0; }
+// Values are kept in [0, P61]: P61 is a second representation of 0, which
+// lets sums and products be reduced by folding (x mod 2^61 + x >> 61) alone.
+M2_HD uint64_t fold61(uint64_t x) { return (x & P61) + (x >> 61); }
+M2_HD uint64_t add61(uint64_t x, uint64_t y) { return fold61(x + y); }
+M2_HD uint64_t neg61(uint64_t x) { return x ^ P61; }  // P61 - x
+M2_HD uint64_t sub61(uint64_t x, uint64_t y) { return fold61(x + (y ^ P61)); }
+// Unreduced pieces of a 61x61-bit product: with 32-bit limbs x = x1 2^32 + x0,
+// x y = p11 2^64 + p01 2^32 + p00, and 2^64 = 2^3, 2^61 = 1 (mod P61).
+M2_HD uint64_t fold_product61(uint64_t p00, uint64_t p01, uint64_t p11) {
+  const uint64_t r = (p11 << 3) + (p00 & P61) + (p00 >> 61) + (p01 >> 29) + ((p01 & 0x1FFFFFFFull) << 32);
+  return fold61(fold61(r));
+}
 M2_HD uint64_t mul61(uint64_t x, uint64_t y) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  // 32-bit limbs: x = x1*2^32 + x0 with x1 < 2^29. Each partial product is one
-  // v_mad_u64_u32; 2^64 = 2^3 and 2^61 = 1 (mod M61) fold them directly.
+  // one v_mad_u64_u32 per partial product
   const uint32_t x0 = uint32_t(x), x1 = uint32_t(x >> 32);
   const uint32_t y0 = uint32_t(y), y1 = uint32_t(y >> 32);
-  const uint64_t p00 = uint64_t(x0) * y0;
-  const uint64_t p01 = uint64_t(x0) * y1 + uint64_t(x1) * y0;  // < 2^62
-  const uint64_t p11 = uint64_t(x1 << 3) * y1;                   // x1*y1*2^3 < 2^61
-  // p01 * 2^32 = (p01 >> 29) * 2^61 + (p01 mod 2^29) * 2^32
-  uint64_t r = (p00 & P61) + (p00 >> 61) + p11 + (p01 >> 29) + ((p01 & 0x1FFFFFFFull) << 32);
-  r = (r & P61) + (r >> 61);
-  return r >= P61 ? r - P61 : r;
+  return fold_product61(uint64_t(x0) * y0, uint64_t(x0) * y1 + uint64_t(x1) * y0, uint64_t(x1) * y1);
 #else
   unsigned __int128 t = (unsigned __int128)x * y;
-  uint64_t r = (uint64_t(t) & P61) + uint64_t(t >> 61);
-  r = (r & P61) + (r >> 61);
-  return r >= P61 ? r - P61 : r;
+  return fold61(fold61((uint64_t(t) & P61) + uint64_t(t >> 61)));
+#endif
+}
+// a b + c d (mod P61) with the two products' pieces summed before folding.
+M2_HD uint64_t dot61(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32), b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+  const uint32_t c0 = uint32_t(c), c1 = uint32_t(c >> 32), d0 = uint32_t(d), d1 = uint32_t(d >> 32);
+  const uint64_t lo = fold61(uint64_t(a0) * b0) + fold61(uint64_t(c0) * d0);       // < 2^62 + 16
+  const uint64_t mid = uint64_t(a0) * b1 + uint64_t(a1) * b0;                          // < 2^62
+  const uint64_t mid2 = uint64_t(c0) * d1 + uint64_t(c1) * d0;                         // < 2^62
+  const uint64_t hi = uint64_t(a1) * b1 + uint64_t(c1) * d1;                           // < 2^59
+  const uint64_t r = lo + (hi << 3) + (mid >> 29) + ((mid & 0x1FFFFFFFull) << 32) + (mid2 >> 29) +
+                     ((mid2 & 0x1FFFFFFFull) << 32);                                    // < 2^64
+  return fold61(fold61(r));
+#else
+  return add61(mul61(a, b), mul61(c, d));
 #endif
 }
 // x * 2^s for 0 <= s < 61: rotate the 61-bit word left by s.
 M2_HD uint64_t rot61(uint64_t x, uint32_t s) { return ((x << s) & P61) | (x >> (61u - s)); }
 
 // ---- M31 ----
-M2_HD uint32_t add31(uint32_t x, uint32_t y) { uint32_t t = x + y; return t >= P31 ? t - P31 : t; }
-M2_HD uint32_t sub31(uint32_t x, uint32_t y) { return x >= y ? x - y : x + P31 - y; }
-M2_HD uint32_t neg31(uint32_t x) { return x ? P31 - x : 0; }
+// Also in [0, P31] with P31 as a second 0; the unsigned min picks the
+// reduced one of t and t -/+ P31.
+M2_HD uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+M2_HD uint32_t add31(uint32_t x, uint32_t y) { const uint32_t t = x + y; return umin32(t, t - P31); }
+M2_HD uint32_t sub31(uint32_t x, uint32_t y) { const uint32_t t = x - y; return umin32(t, t + P31); }
+M2_HD uint32_t neg31(uint32_t x) { return P31 - x; }
 M2_HD uint32_t mul31(uint32_t x, uint32_t y) {
   const uint64_t t = uint64_t(x) * y;
   const uint32_t r = (uint32_t(t) & P31) + uint32_t(t >> 31);
-  return r >= P31 ? r - P31 : r;
+  return umin32(r, r - P31);
 }
 M2_HD uint32_t rot31(uint32_t x, uint32_t s) {
   const uint64_t t = uint64_t(x) << s;
@@ -70,18 +91,11 @@ M2_HD e61 sub(e61 x, e61 y) { return {sub61(x.a, y.a), sub61(x.b, y.b)}; }
 M2_HD e61 conj(e61 x) { return {x.a, neg61(x.b)}; }
 M2_HD e61 muli(e61 x) { return {neg61(x.b), x.a}; }   // x * i
 M2_HD e61 mulmi(e61 x) { return {x.b, neg61(x.a)}; }  // x * -i
-// Karatsuba: 3 base products.
-M2_HD e61 mul(e61 x, e61 y) {
-  const uint64_t ac = mul61(x.a, y.a), bd = mul61(x.b, y.b);
-  const uint64_t k = mul61(add61(x.a, x.b), add61(y.a, y.b));
-  return {sub61(ac, bd), sub61(sub61(k, ac), bd)};
-}
-// x * conj(y)
-M2_HD e61 mulconj(e61 x, e61 y) {
-  const uint64_t ac = mul61(x.a, y.a), bd = mul61(x.b, y.b);
-  const uint64_t k = mul61(add61(x.a, x.b), sub61(y.a, y.b));
-  return {add61(ac, bd), add61(sub61(k, ac), bd)};
-}
+// (a + bi)(c + di) = (ac + (P-b) d) + (ad + bc) i: each component is one dot
+// product, whose four 64-bit partial products share two folds.
+M2_HD e61 mul(e61 x, e61 y) { return {dot61(x.a, y.a, neg61(x.b), y.b), dot61(x.a, y.b, x.b, y.a)}; }
+// x * conj(y) = (ac + bd) + (bc - ad) i
+M2_HD e61 mulconj(e61 x, e61 y) { return {dot61(x.a, y.a, x.b, y.b), dot61(x.b, y.a, neg61(x.a), y.b)}; }
 M2_HD e61 sqr(e61 x) {
   const uint64_t t = mul61(x.a, x.b);
   return {mul61(add61(x.a, x.b), sub61(x.a, x.b)), add61(t, t)};

@@ -14,8 +14,9 @@ namespace m2 {
 // Device-side view of one test: buffers and the pass plan, passed by value
 // to the launchers.
 struct DevPlan {
-  e61* z61 = nullptr;  // h elements: spectrum (M61 part) / digits as int64 pairs
+  e61* z61 = nullptr;  // h elements: spectrum (M61 part); small plan: also the digits
   e31* z31 = nullptr;  // h elements: spectrum (M31 part)
+  e61* zd = nullptr;   // multi-pass plan: digits as int64 pairs, strip-major (h slots)
   const e61* tw61 = nullptr;       // step twiddle tables
   const e31* tw31 = nullptr;
   const e61* sq61 = nullptr;  // centre twiddles, h/2 entries

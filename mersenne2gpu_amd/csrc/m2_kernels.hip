@@ -1,12 +1,13 @@
 // gfx950 HIP kernels for one IBDWT squaring of x mod 2^p - 1 over
 // GF(M61^2) x GF(M31^2).
 //
-// Data between iterations: z61 holds the digits as signed 64-bit pairs
-// (digits 2e and 2e+1 of element e) and G holds the carries not yet applied.
-// On the multi-pass plan the digits and G are stored strip-major (all rows of
-// one outer-pass column strip contiguous), so the outer passes read and write
-// them with full cache lines. During an iteration z61/z31 hold the weighted
-// spectrum in natural element order.
+// Data between iterations: the digits as signed 64-bit pairs (digits 2e and
+// 2e+1 of element e) and G, the carries not yet applied. On the small plan the
+// digits live in z61 in natural order. On the multi-pass plan they live in
+// their own buffer zd, strip-major (all rows of one outer-pass column strip
+// contiguous) like G, so the outer passes read and write them with full cache
+// lines; a separate buffer, because the spectrum in z61/z31 is in natural
+// element order and a workgroup's strip overlaps other workgroups' rows.
 //
 // Carry rule (exact, no waits between workgroups): the digits are cut into
 // segments; every segment is normalised with carry-in 0, then receives the
@@ -398,8 +399,9 @@ __device__ __forceinline__ uint32_t prev_strip_carry(uint32_t cg, uint32_t r, ui
 
 constexpr int OUTER_TW = PASS_MAX_ELEMS / 2;  // R - 1 twiddles per block, R <= 512 here
 
-__global__ void __launch_bounds__(256) k_outer_fwd(e61* __restrict__ z61, e31* __restrict__ z31,
-                                                   const e61* __restrict__ tw61, const e31* __restrict__ tw31,
+__global__ void __launch_bounds__(256) k_outer_fwd(const e61* __restrict__ zd, e61* __restrict__ z61,
+                                                   e31* __restrict__ z31, const e61* __restrict__ tw61,
+                                                   const e31* __restrict__ tw31,
                                                    const Carry128* __restrict__ G, Pass ps, int logS, int logW,
                                                    DigitGen D) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
@@ -411,7 +413,7 @@ __global__ void __launch_bounds__(256) k_outer_fwd(e61* __restrict__ z61, e31* _
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
   // digits of the strip are contiguous
-  const e61* src = z61 + (size_t)cg * (R * W);
+  const e61* src = zd + (size_t)cg * (R * W);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) L61[PADX(i)] = src[i];
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
@@ -433,8 +435,9 @@ __global__ void __launch_bounds__(256) k_outer_fwd(e61* __restrict__ z61, e31* _
   }
 }
 
-__global__ void __launch_bounds__(256) k_outer_inv(e61* __restrict__ z61, const e31* __restrict__ z31,
-                                                   const e61* __restrict__ tw61, const e31* __restrict__ tw31,
+__global__ void __launch_bounds__(256) k_outer_inv(const e61* __restrict__ z61, const e31* __restrict__ z31,
+                                                   e61* __restrict__ zd, const e61* __restrict__ tw61,
+                                                   const e31* __restrict__ tw31,
                                                    Carry128* __restrict__ G, Pass ps, int logS, int logW, DigitGen D,
                                                    int mode) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
@@ -471,7 +474,7 @@ __global__ void __launch_bounds__(256) k_outer_inv(e61* __restrict__ z61, const 
     if (v == subs - 1) G[cg * (uint32_t)R + (uint32_t)r] = gs[it];
   }
   __syncthreads();
-  e61* dst = z61 + (size_t)cg * (R * W);
+  e61* dst = zd + (size_t)cg * (R * W);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) dst[i] = L61[PADX(i)];
 }
 
@@ -564,8 +567,8 @@ template <class F>
 hipError_t iteration(const DevPlan& P, hipStream_t st, F&& mark) {
   hipError_t e;
   const uint32_t ngroups_outer = (1u << P.logS) >> P.logW;
-  hipLaunchKernelGGL(k_outer_fwd, dim3(ngroups_outer), dim3(256), 0, st, P.z61, P.z31, P.tw61, P.tw31, P.G, P.outer,
-                     P.logS, P.logW, P.dgen);
+  hipLaunchKernelGGL(k_outer_fwd, dim3(ngroups_outer), dim3(256), 0, st, P.zd, P.z61, P.z31, P.tw61, P.tw31, P.G,
+                     P.outer, P.logS, P.logW, P.dgen);
   if ((e = mark("k_outer_fwd")) != hipSuccess) return e;
   const uint32_t nmid = P.has_mid ? (1u << P.outer.A) * ((1u << P.mid_logS) >> P.mid_logW) : 0;
   if (P.has_mid) {
@@ -582,8 +585,8 @@ hipError_t iteration(const DevPlan& P, hipStream_t st, F&& mark) {
                        P.mid_logW);
     if ((e = mark("k_mid_inv")) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_outer_inv, dim3(ngroups_outer), dim3(256), 0, st, P.z61, P.z31, P.tw61, P.tw31, P.G, P.outer,
-                     P.logS, P.logW, P.dgen, P.mode);
+  hipLaunchKernelGGL(k_outer_inv, dim3(ngroups_outer), dim3(256), 0, st, P.z61, P.z31, P.zd, P.tw61, P.tw31, P.G,
+                     P.outer, P.logS, P.logW, P.dgen, P.mode);
   if ((e = mark("k_outer_inv")) != hipSuccess) return e;
   return hipGetLastError();
 }
