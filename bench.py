@@ -91,6 +91,8 @@ def main() -> int:
     ap.add_argument("--kernel-iters", type=int, default=200, help="iterations timed kernel by kernel")
     ap.add_argument("--cpu-iters", type=int, default=9, help="oracle iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device-to-device 3 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -98,6 +100,16 @@ def main() -> int:
         print("bench.py runs one test on one GPU; multi-GPU runs are not part of this build "
               "(independent tests shard across GPUs with no communication)", file=sys.stderr)
         return 2
+
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 3 x 256 MiB device-to-device")
 
     import mersenne2gpu_amd as m2
     from mersenne2gpu_amd import workloads as W
@@ -139,7 +151,7 @@ def main() -> int:
     us_per_iter = ms * 1000.0 / args.steps
     log("timed %d iterations: %.3f ms device, %.3f s wall -> %.3f us/iter" % (args.steps, ms, wall, us_per_iter))
 
-    # per-kernel device time (event after each kernel)
+    # per-kernel device time (each kernel's dispatch start/stop timestamps)
     kt = t.kernel_times(args.kernel_iters)
     per_kernel_us = {k: v * 1000.0 / args.kernel_iters for k, v in kt.items()}
     log("per-kernel us:", json.dumps({k: round(v, 3) for k, v in per_kernel_us.items()}))

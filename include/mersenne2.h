@@ -100,9 +100,10 @@ int m2_test_is_prime(m2_test *t, int *verdict);
 /* Run `iters` iterations and return their wall time on the device in ms
  * (hipEvents around the launches). */
 int m2_test_time(m2_test *t, uint64_t iters, double *ms);
-/* Run `iters` iterations kernel by kernel with an event between kernels and
- * accumulate per-kernel device time. names_buf receives the kernel names
- * separated by '\n'; ms[i] the total milliseconds of kernel i. */
+/* Run `iters` iterations kernel by kernel, recording each kernel's dispatch
+ * start/stop timestamps, and accumulate per-kernel device time. names_buf
+ * receives the kernel names separated by '\n'; ms[i] the total milliseconds
+ * of kernel i. */
 int m2_test_kernel_times(m2_test *t, uint64_t iters, double *ms, int max_kernels, int *count, char *names_buf,
                          size_t names_len);
 /* Human-readable plan (the reference's --debug-plan, mersenne2_gpu.cpp:4767-4776). */

@@ -374,30 +374,34 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
   if (!t || !ms || !count) return fail(M2_E_INVAL, "null pointer");
   HIPCHK(hipSetDevice(t->ctx->device));
   constexpr int MAXE = 8;
-  hipEvent_t ev[MAXE + 1];
-  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  hipEvent_t starts[MAXE], stops[MAXE];
+  for (int k = 0; k < MAXE; ++k) {
+    HIPCHK(hipEventCreate(&starts[k]));
+    HIPCHK(hipEventCreate(&stops[k]));
+  }
   const char* names[MAXE] = {nullptr};
   int nk = 0;
   std::vector<double> acc(MAXE, 0.0);
   for (uint64_t it = 0; it < iters; ++it) {
-    HIPCHK(hipEventRecord(ev[0], t->stream));
     if (t->P.small) {
-      HIPCHK(launch_small(t->P, 1, t->stream));
-      HIPCHK(hipEventRecord(ev[1], t->stream));
+      HIPCHK(launch_small_events(t->P, 1, t->stream, starts[0], stops[0]));
       nk = 1;
       names[0] = "k_small";
     } else {
-      HIPCHK(launch_iteration_events(t->P, t->stream, ev + 1, MAXE, &nk, names));
+      HIPCHK(launch_iteration_events(t->P, t->stream, starts, stops, MAXE, &nk, names));
     }
     ++t->iter;
-    HIPCHK(hipEventSynchronize(ev[nk]));
+    HIPCHK(hipEventSynchronize(stops[nk - 1]));
     for (int k = 0; k < nk; ++k) {
       float f = 0;
-      HIPCHK(hipEventElapsedTime(&f, ev[k], ev[k + 1]));
+      HIPCHK(hipEventElapsedTime(&f, starts[k], stops[k]));
       acc[k] += f;
     }
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < MAXE; ++k) {
+    (void)hipEventDestroy(starts[k]);
+    (void)hipEventDestroy(stops[k]);
+  }
   const int m = std::min(nk, max_kernels);
   std::string nm;
   for (int k = 0; k < m; ++k) {

@@ -58,10 +58,11 @@ inline uint64_t carry_target_digit(const DevPlan& P, uint64_t g) {
 
 hipError_t launch_small(const DevPlan& P, uint32_t iters, hipStream_t st);
 hipError_t launch_iteration(const DevPlan& P, hipStream_t st);
-// Kernel-by-kernel launch of one iteration with an event after each kernel
-// (events[0] must be recorded by the caller before); returns kernel names.
-hipError_t launch_iteration_events(const DevPlan& P, hipStream_t st, hipEvent_t* events, int max_events, int* count,
-                                   const char** names);
+// One iteration with each kernel's own dispatch timestamps recorded in
+// starts[k]/stops[k] (hipExtLaunchKernelGGL); returns the kernel names.
+hipError_t launch_iteration_events(const DevPlan& P, hipStream_t st, hipEvent_t* starts, hipEvent_t* stops,
+                                   int max_events, int* count, const char** names);
+hipError_t launch_small_events(const DevPlan& P, uint32_t iters, hipStream_t st, hipEvent_t start, hipEvent_t stop);
 
 // ---- host side (m2_host.cpp) ----
 int transform_log2(uint64_t p);

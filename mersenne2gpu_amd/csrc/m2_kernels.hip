@@ -21,6 +21,8 @@
 // Twiddles: every workgroup first copies the twiddle blocks of all its radix
 // steps (and of the centre's squaring) into LDS with coalesced loads, so no
 // radix step waits on global memory.
+#include <hip/hip_ext.h>
+
 #include "m2_arith.h"
 #include "m2_digits.h"
 #include "m2_internal.h"
@@ -562,50 +564,55 @@ hipError_t launch_small(const DevPlan& P, uint32_t iters, hipStream_t st) {
 }
 
 namespace {
-// Launches the kernels of one iteration; calls mark(name) after each.
+// Launches the kernels of one iteration through launch(name, kernel, grid, args...).
 template <class F>
-hipError_t iteration(const DevPlan& P, hipStream_t st, F&& mark) {
+hipError_t iteration(const DevPlan& P, F&& launch) {
   hipError_t e;
   const uint32_t ngroups_outer = (1u << P.logS) >> P.logW;
-  hipLaunchKernelGGL(k_outer_fwd, dim3(ngroups_outer), dim3(256), 0, st, P.zd, P.z61, P.z31, P.tw61, P.tw31, P.G,
-                     P.outer, P.logS, P.logW, P.dgen);
-  if ((e = mark("k_outer_fwd")) != hipSuccess) return e;
+  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ngroups_outer), P.zd, P.z61, P.z31, P.tw61, P.tw31,
+                  (const Carry128*)P.G, P.outer, P.logS, P.logW, P.dgen)) != hipSuccess)
+    return e;
   const uint32_t nmid = P.has_mid ? (1u << P.outer.A) * ((1u << P.mid_logS) >> P.mid_logW) : 0;
-  if (P.has_mid) {
-    hipLaunchKernelGGL(k_mid<true>, dim3(nmid), dim3(256), 0, st, P.z61, P.z31, P.tw61, P.tw31, P.mid, P.mid_logS,
-                       P.mid_logW);
-    if ((e = mark("k_mid_fwd")) != hipSuccess) return e;
-  }
+  if (P.has_mid && (e = launch("k_mid_fwd", k_mid<true>, dim3(nmid), P.z61, P.z31, P.tw61, P.tw31, P.mid,
+                               P.mid_logS, P.mid_logW)) != hipSuccess)
+    return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
-  hipLaunchKernelGGL(k_centre, dim3(ncentre), dim3(256), 0, st, P.z61, P.z31, P.tw61, P.tw31, P.sq61, P.sq31,
-                     P.centre);
-  if ((e = mark("k_centre")) != hipSuccess) return e;
-  if (P.has_mid) {
-    hipLaunchKernelGGL(k_mid<false>, dim3(nmid), dim3(256), 0, st, P.z61, P.z31, P.tw61, P.tw31, P.mid, P.mid_logS,
-                       P.mid_logW);
-    if ((e = mark("k_mid_inv")) != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(k_outer_inv, dim3(ngroups_outer), dim3(256), 0, st, P.z61, P.z31, P.zd, P.tw61, P.tw31, P.G,
-                     P.outer, P.logS, P.logW, P.dgen, P.mode);
-  if ((e = mark("k_outer_inv")) != hipSuccess) return e;
-  return hipGetLastError();
+  if ((e = launch("k_centre", k_centre, dim3(ncentre), P.z61, P.z31, P.tw61, P.tw31, P.sq61, P.sq31, P.centre)) !=
+      hipSuccess)
+    return e;
+  if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid), P.z61, P.z31, P.tw61, P.tw31, P.mid,
+                               P.mid_logS, P.mid_logW)) != hipSuccess)
+    return e;
+  return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer), (const e61*)P.z61, (const e31*)P.z31, P.zd, P.tw61,
+                P.tw31, P.G, P.outer, P.logS, P.logW, P.dgen, P.mode);
 }
 }  // namespace
 
 hipError_t launch_iteration(const DevPlan& P, hipStream_t st) {
-  return iteration(P, st, [](const char*) { return hipSuccess; });
+  return iteration(P, [&](const char*, auto kernel, dim3 grid, auto... args) -> hipError_t {
+    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, args...);
+    return hipGetLastError();
+  });
 }
 
-hipError_t launch_iteration_events(const DevPlan& P, hipStream_t st, hipEvent_t* ev, int max_events, int* count,
-                                   const char** names) {
+hipError_t launch_iteration_events(const DevPlan& P, hipStream_t st, hipEvent_t* starts, hipEvent_t* stops,
+                                   int max_events, int* count, const char** names) {
   int k = 0;
-  hipError_t e = iteration(P, st, [&](const char* name) -> hipError_t {
+  hipError_t e = iteration(P, [&](const char* name, auto kernel, dim3 grid, auto... args) -> hipError_t {
     if (k >= max_events) return hipErrorInvalidValue;
     names[k] = name;
-    return hipEventRecord(ev[k++], st);
+    hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, st, starts[k], stops[k], 0, args...);
+    ++k;
+    return hipGetLastError();
   });
   *count = k;
   return e;
+}
+
+hipError_t launch_small_events(const DevPlan& P, uint32_t iters, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
+  hipExtLaunchKernelGGL(k_small, dim3(1), dim3(256), 0, st, start, stop, 0, P.z61, P.tw61, P.tw31, P.sq61, P.sq31,
+                        P.outer, P.dgen, P.mode, iters);
+  return hipGetLastError();
 }
 
 }  // namespace m2
