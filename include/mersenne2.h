@@ -109,6 +109,55 @@ int m2_test_kernel_times(m2_test *t, uint64_t iters, double *ms, int max_kernels
 /* Human-readable plan (the reference's --debug-plan, mersenne2_gpu.cpp:4767-4776). */
 int m2_test_plan(m2_test *t, char *buf, size_t len);
 
+/*
+ * Batches: many tests, of any mix of exponents, resident on one device and
+ * advanced together. Tests of the same transform length share every launch
+ * (one workgroup per test on single-workgroup lengths, one grid row per test
+ * on multi-pass lengths); lengths run on separate streams. The reference runs
+ * one exponent per process (mersenne2_gpu.cpp:3618-4960); a batch gives each
+ * test the result that process would compute for it.
+ */
+
+/* Create `count` tests (ps[i], modes[i]); iters[i] is the number of
+ * iterations test i stops at (0: its full test, p or p - 2). iters may be NULL. */
+int m2_batch_create(m2_ctx *ctx, size_t count, const uint64_t *ps, const int *modes, const uint64_t *iters,
+                    const m2_opts *opts, m2_batch **out);
+int m2_batch_destroy(m2_batch *b);
+/* Enqueue up to `max_iters` more iterations of every unfinished test (0: run
+ * every test to its stop) and return. */
+int m2_batch_run(m2_batch *b, uint64_t max_iters);
+int m2_batch_sync(m2_batch *b);
+/* Iterations test i has had enqueued. */
+int m2_batch_iteration(m2_batch *b, size_t i, uint64_t *iter);
+/* Res64, residue and verdict of test i (these wait for test i's stream). */
+int m2_batch_res64(m2_batch *b, size_t i, uint64_t *res64);
+int m2_batch_residue(m2_batch *b, size_t i, uint8_t *buf, size_t len);
+int m2_batch_is_prime(m2_batch *b, size_t i, int *verdict);
+
+/*
+ * Server: a request queue over resident tests. A request is admitted, in
+ * arrival order, once its device footprint (m2_test_footprint) fits in the
+ * budget; each m2_server_step gives every admitted test one time slice of
+ * `slice_iters` iterations, then retires finished tests (their Res64 and
+ * verdict go to the completion queue read by m2_server_poll) and frees their
+ * memory. No counterpart in the reference, which tests one exponent per run.
+ * m2_server_submit / _poll / _stats may be called from other threads while
+ * one thread drives m2_server_step.
+ */
+typedef struct m2_server m2_server;
+/* hbm_budget_bytes 0: 70% of the device memory; slice_iters 0: 1000. */
+int m2_server_create(m2_ctx *ctx, uint64_t hbm_budget_bytes, uint32_t slice_iters, const m2_opts *opts,
+                     m2_server **out);
+int m2_server_destroy(m2_server *s);
+/* Queue a test of 2^p - 1; iters 0 means the full test, else the test stops
+ * after `iters` iterations (its verdict is then that of the partial state). */
+int m2_server_submit(m2_server *s, uint64_t p, int mode, uint64_t iters, uint64_t *id);
+/* Admit, run one slice, retire. *busy = 1 while requests remain queued or active. */
+int m2_server_step(m2_server *s, int *busy);
+/* Take up to `max` completions (any of ids/res64/verdicts may be NULL). */
+int m2_server_poll(m2_server *s, uint64_t *ids, uint64_t *res64, int *verdicts, size_t max, size_t *count);
+int m2_server_stats(m2_server *s, uint64_t *queued, uint64_t *active, uint64_t *bytes_in_use);
+
 #ifdef __cplusplus
 }
 #endif

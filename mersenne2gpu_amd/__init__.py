@@ -48,6 +48,23 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "m2_test_kernel_times": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double), _c.c_int, _c.POINTER(_c.c_int),
                                         _c.c_char_p, _c.c_size_t]),
     "m2_test_plan": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
+    "m2_batch_create": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
+                                   _c.POINTER(_c.c_uint64), _P, _c.POINTER(_P)]),
+    "m2_batch_destroy": (_c.c_int, [_P]),
+    "m2_batch_run": (_c.c_int, [_P, _c.c_uint64]),
+    "m2_batch_sync": (_c.c_int, [_P]),
+    "m2_batch_iteration": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_c.c_uint64)]),
+    "m2_batch_res64": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_c.c_uint64)]),
+    "m2_batch_residue": (_c.c_int, [_P, _c.c_size_t, _c.c_char_p, _c.c_size_t]),
+    "m2_batch_is_prime": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_c.c_int)]),
+    "m2_server_create": (_c.c_int, [_P, _c.c_uint64, _c.c_uint32, _P, _c.POINTER(_P)]),
+    "m2_server_destroy": (_c.c_int, [_P]),
+    "m2_server_submit": (_c.c_int, [_P, _c.c_uint64, _c.c_int, _c.c_uint64, _c.POINTER(_c.c_uint64)]),
+    "m2_server_step": (_c.c_int, [_P, _c.POINTER(_c.c_int)]),
+    "m2_server_poll": (_c.c_int, [_P, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
+                                  _c.c_size_t, _c.POINTER(_c.c_size_t)]),
+    "m2_server_stats": (_c.c_int, [_P, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
+                                   _c.POINTER(_c.c_uint64)]),
 }
 
 _LIB = None
@@ -230,6 +247,139 @@ class Test:
         buf = ctypes.create_string_buffer(4096)
         _chk(lib().m2_test_plan(self.h, buf, 4096), "m2_test_plan")
         return buf.value.decode()
+
+
+def _mode(m) -> int:
+    if m in ("prp", MODE_PRP):
+        return MODE_PRP
+    if m in ("ll", MODE_LL):
+        return MODE_LL
+    raise ValueError("mode must be 'll' or 'prp': %r" % (m,))
+
+
+class Batch:
+    """Many resident tests advanced together (mirror of m2_batch_*).
+
+    iters[i] (optional) is the iteration test i stops at; 0 or None means its
+    full test (p PRP iterations, p - 2 LL steps)."""
+
+    def __init__(self, ctx: Context, ps: Sequence[int], modes, iters: Optional[Sequence[int]] = None,
+                 local_stage_cap: int = 0):
+        n = len(ps)
+        if isinstance(modes, str):
+            modes = [modes] * n
+        self.ps = [int(p) for p in ps]
+        self.modes = list(modes)
+        P = (ctypes.c_uint64 * max(n, 1))(*self.ps)
+        M = (ctypes.c_int * max(n, 1))(*[_mode(m) for m in self.modes])
+        It = None
+        if iters is not None:
+            It = (ctypes.c_uint64 * max(n, 1))(*[int(k or 0) for k in iters])
+        opts = _Opts()
+        opts.local_stage_cap = local_stage_cap
+        h = ctypes.c_void_p()
+        _chk(lib().m2_batch_create(ctx.h, n, P, M, It, ctypes.byref(opts), ctypes.byref(h)), "m2_batch_create")
+        self.h = h
+
+    def __len__(self) -> int:
+        return len(self.ps)
+
+    def close(self) -> None:
+        if self.h:
+            lib().m2_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, max_iters: int = 0) -> None:
+        _chk(lib().m2_batch_run(self.h, int(max_iters)), "m2_batch_run")
+
+    def sync(self) -> None:
+        _chk(lib().m2_batch_sync(self.h), "m2_batch_sync")
+
+    def iteration(self, i: int) -> int:
+        v = ctypes.c_uint64()
+        _chk(lib().m2_batch_iteration(self.h, i, ctypes.byref(v)), "m2_batch_iteration")
+        return v.value
+
+    def res64(self, i: int) -> str:
+        v = ctypes.c_uint64()
+        _chk(lib().m2_batch_res64(self.h, i, ctypes.byref(v)), "m2_batch_res64")
+        return "%016X" % v.value
+
+    def residue(self, i: int) -> bytes:
+        n = (self.ps[i] + 7) // 8
+        buf = ctypes.create_string_buffer(n)
+        _chk(lib().m2_batch_residue(self.h, i, buf, n), "m2_batch_residue")
+        return buf.raw[:n]
+
+    def is_prime(self, i: int) -> bool:
+        v = ctypes.c_int()
+        _chk(lib().m2_batch_is_prime(self.h, i, ctypes.byref(v)), "m2_batch_is_prime")
+        return bool(v.value)
+
+
+class Server:
+    """Request queue over resident tests (mirror of m2_server_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int = 0, slice_iters: int = 0, local_stage_cap: int = 0):
+        opts = _Opts()
+        opts.local_stage_cap = local_stage_cap
+        h = ctypes.c_void_p()
+        _chk(lib().m2_server_create(ctx.h, int(hbm_budget_bytes), int(slice_iters), ctypes.byref(opts),
+                                    ctypes.byref(h)), "m2_server_create")
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib().m2_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, p: int, mode="prp", iters: int = 0) -> int:
+        v = ctypes.c_uint64()
+        _chk(lib().m2_server_submit(self.h, int(p), _mode(mode), int(iters), ctypes.byref(v)), "m2_server_submit")
+        return v.value
+
+    def step(self) -> bool:
+        """One admit / slice / retire round; True while work remains."""
+        v = ctypes.c_int()
+        _chk(lib().m2_server_step(self.h, ctypes.byref(v)), "m2_server_step")
+        return bool(v.value)
+
+    def poll(self, max_items: int = 4096) -> List[Tuple[int, str, bool]]:
+        ids = (ctypes.c_uint64 * max_items)()
+        r64 = (ctypes.c_uint64 * max_items)()
+        ver = (ctypes.c_int * max_items)()
+        cnt = ctypes.c_size_t()
+        _chk(lib().m2_server_poll(self.h, ids, r64, ver, max_items, ctypes.byref(cnt)), "m2_server_poll")
+        return [(ids[i], "%016X" % r64[i], bool(ver[i])) for i in range(cnt.value)]
+
+    def stats(self) -> Dict[str, int]:
+        q, a, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _chk(lib().m2_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(b)), "m2_server_stats")
+        return {"queued": q.value, "active": a.value, "bytes_in_use": b.value}
 
 
 def run_checkpoints(ctx: Context, p: int, mode: str, checkpoints: Sequence[int]) -> Dict[int, str]:

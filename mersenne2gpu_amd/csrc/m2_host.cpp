@@ -51,8 +51,8 @@ uint32_t bitrev(uint32_t j, int bits) {
 int log2i(uint32_t r) { return r == 8 ? 3 : (r == 4 ? 2 : 1); }
 
 // Levels [ell0, ell0 + A): one radix-2 or radix-4 step for A mod 3 levels,
-// then radix-8 steps. Twiddle blocks are appended to T.
-Pass make_pass(int ell0, int A, HostTables& T) {
+// then radix-8 steps. Twiddle blocks are appended to tw61/tw31.
+Pass make_pass(int ell0, int A, std::vector<e61>& tw61, std::vector<e31>& tw31) {
   Pass ps{};
   ps.ell0 = ell0;
   ps.A = A;
@@ -61,7 +61,7 @@ Pass make_pass(int ell0, int A, HostTables& T) {
     Step& s = ps.steps[ps.nsteps++];
     s.ell = ell;
     s.radix = radix;
-    s.twoff = (uint32_t)T.tw61.size();
+    s.twoff = (uint32_t)tw61.size();
     const int lr = log2i(radix);
     // tau_j = zeta of block 2^lr * j at level ell + lr = w_{2^(ell+lr)}^bitrev(j)
     const uint32_t nb = 1u << ell;
@@ -84,8 +84,8 @@ Pass make_pass(int ell0, int A, HostTables& T) {
       e61 a61 = t61;
       e31 a31 = t31;
       for (int e = 1; e < radix; ++e) {
-        T.tw61.push_back(a61);
-        T.tw31.push_back(a31);
+        tw61.push_back(a61);
+        tw31.push_back(a31);
         a61 = mul(a61, t61);
         a31 = mul(a31, t31);
       }
@@ -101,77 +101,16 @@ Pass make_pass(int ell0, int A, HostTables& T) {
 
 }  // namespace
 
-bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T, std::string& err) {
-  const int ln = transform_log2(p);
-  if (ln > 27) {
-    err = "exponent too large: transform length above 2^27";
-    return false;
-  }
-  if ((p & 1u) == 0) {
-    err = "exponent must be odd";
+bool build_class_plan(int ln, uint32_t ls_cap, DevPlan& P, ClassTables& T, std::string& err) {
+  if (ln < 3 || ln > 27) {
+    err = "transform length 2^" + std::to_string(ln) + " outside 2^3..2^27";
     return false;
   }
   const uint64_t n = uint64_t(1) << ln, h = n / 2;
-  const uint32_t q = p >> ln;
+  P = DevPlan{};
   P.ln = ln;
   P.L = ln - 1;
   P.h = (uint32_t)h;
-  P.mode = mode;
-
-  // Digit widths ceil(p(j+1)/n) - ceil(pj/n) and weight shifts
-  // lr*(n - (pj mod n)) with 2^lr an n-th root of 2 in each field
-  // (mersenne2_gpu.cpp:3921-3944).
-  T.dig.assign(n, DigitInfo{0, 0, 0, 0});
-  T.bitpos.assign(n + 1, 0);
-  const uint64_t lr61 = ((uint64_t(1) << 60) >> ln) % 61u;
-  const uint64_t lr31 = ((uint64_t(1) << 30) >> ln) % 31u;
-  for (uint64_t j = 0; j <= n; ++j) T.bitpos[j] = j ? (uint32_t)(((uint64_t)p * j - 1) / n + 1) : 0u;
-  for (uint64_t j = 0; j < n; ++j) {
-    const uint32_t wd = T.bitpos[j + 1] - T.bitpos[j];
-    if (wd != q && wd != q + 1) {
-      err = "digit width generation error";
-      return false;
-    }
-    T.dig[j].width = (uint8_t)wd;
-    if (j) {
-      const uint64_t r = ((uint64_t)p * j) & (n - 1);
-      T.dig[j].w61 = (uint8_t)((lr61 * (n - r)) % 61u);
-      T.dig[j].w31 = (uint8_t)((lr31 * (n - r)) % 31u);
-    }
-  }
-
-  DigitGen& D = P.dgen;
-  D.p = p;
-  D.nmask = (uint32_t)(n - 1);
-  D.s = (uint32_t)(p & (n - 1));
-  D.q = q;
-  D.lr61 = (uint32_t)lr61;
-  D.lr31 = (uint32_t)lr31;
-  D.d61 = (uint32_t)((lr61 * D.s) % 61u);
-  D.d31 = (uint32_t)((lr31 * D.s) % 31u);
-  D.k61 = (uint32_t)((ln + 1) % 61);
-  D.k31 = (uint32_t)((ln + 1) % 31);
-
-  // The kernels walk widths and weights incrementally; check the walk against
-  // the tables above for every digit.
-  {
-    DWalk w = dwalk_at(D, 0);
-    for (uint64_t j = 0; j < n; ++j) {
-      if (j % 4096 == 0) {
-        const DWalk f = dwalk_at(D, (uint32_t)j);
-        if (f.r != w.r || f.v61 != w.v61 || f.v31 != w.v31) {
-          err = "digit walk: restart mismatch";
-          return false;
-        }
-      }
-      if (dwalk_width(D, w) != T.dig[j].width || dwalk_w61(w) != T.dig[j].w61 || dwalk_w31(w) != T.dig[j].w31) {
-        err = "digit walk mismatch at digit " + std::to_string(j);
-        return false;
-      }
-      dwalk_next(D, w);
-    }
-  }
-
   T.tw61.clear();
   T.tw31.clear();
   const int L = P.L;
@@ -186,9 +125,13 @@ bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T
   }
   if (L <= SMALL_LOG_H && !(ls_cap && L >= c + 2)) {
     P.small = true;
-    P.outer = make_pass(0, L, T);
+    P.outer = make_pass(0, L, T.tw61, T.tw31);
     P.nseg = (uint32_t)(h / SEG_ELEMS);
   } else {
+    if (L < c + 2) {
+      err = "plan error: transform too short for the multi-pass plan";
+      return false;
+    }
     P.small = false;
     const int a = L - c;
     int a_outer = a, a_mid = 0;
@@ -196,8 +139,7 @@ bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T
       a_mid = a / 2;
       a_outer = a - a_mid;
     }
-    // the outer pass needs rows of at least one carry segment
-    P.outer = make_pass(0, a_outer, T);
+    P.outer = make_pass(0, a_outer, T.tw61, T.tw31);
     P.logS = L - a_outer;
     P.logW = std::min(11 - a_outer, P.logS);
     if ((1 << P.logW) < SEG_ELEMS) {
@@ -206,18 +148,12 @@ bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T
     }
     if (a_mid) {
       P.has_mid = true;
-      P.mid = make_pass(a_outer, a_mid, T);
+      P.mid = make_pass(a_outer, a_mid, T.tw61, T.tw31);
       P.mid_logS = L - a_outer - a_mid;
       P.mid_logW = std::min(11 - a_mid, P.mid_logS);
     }
-    P.centre = make_pass(L - c, c, T);
+    P.centre = make_pass(L - c, c, T.tw61, T.tw31);
     P.nseg = (uint32_t)(h >> P.logW);
-    P.seg_elems = 1 << P.logW;
-    // The exactness argument for the carry needs 8 digits of at least 12 bits.
-    if (q < 12) {
-      err = "plan error: digit width below 12 bits on the multi-pass plan";
-      return false;
-    }
   }
   // centre twiddles: w_h^bitrev(j) for position 2j (the reference's w[h/2+j])
   T.sq61.resize(h / 2);
@@ -241,6 +177,75 @@ bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T
       T.sq61[j] = p61[br];
       T.sq31[j] = p31[br];
     }
+  }
+  return true;
+}
+
+bool build_test_tables(uint32_t p, const DevPlan& P, TestTables& T, DigitGen& D, std::string& err) {
+  const int ln = P.ln;
+  if (transform_log2(p) != ln) {
+    err = "exponent does not match the plan's transform length";
+    return false;
+  }
+  if ((p & 1u) == 0) {
+    err = "exponent must be odd";
+    return false;
+  }
+  const uint64_t n = uint64_t(1) << ln;
+  const uint32_t q = p >> ln;
+  // The exactness argument for the carry needs 8 digits of at least 12 bits.
+  if (!P.small && q < 12) {
+    err = "plan error: digit width below 12 bits on the multi-pass plan";
+    return false;
+  }
+  // Digit widths ceil(p(j+1)/n) - ceil(pj/n) and weight shifts
+  // lr*(n - (pj mod n)) with 2^lr an n-th root of 2 in each field
+  // (mersenne2_gpu.cpp:3921-3944).
+  T.dig.assign(n, DigitInfo{0, 0, 0, 0});
+  T.bitpos.assign(n + 1, 0);
+  const uint64_t lr61 = ((uint64_t(1) << 60) >> ln) % 61u;
+  const uint64_t lr31 = ((uint64_t(1) << 30) >> ln) % 31u;
+  for (uint64_t j = 0; j <= n; ++j) T.bitpos[j] = j ? (uint32_t)(((uint64_t)p * j - 1) / n + 1) : 0u;
+  for (uint64_t j = 0; j < n; ++j) {
+    const uint32_t wd = T.bitpos[j + 1] - T.bitpos[j];
+    if (wd != q && wd != q + 1) {
+      err = "digit width generation error";
+      return false;
+    }
+    T.dig[j].width = (uint8_t)wd;
+    if (j) {
+      const uint64_t r = ((uint64_t)p * j) & (n - 1);
+      T.dig[j].w61 = (uint8_t)((lr61 * (n - r)) % 61u);
+      T.dig[j].w31 = (uint8_t)((lr31 * (n - r)) % 31u);
+    }
+  }
+  D = DigitGen{};
+  D.p = p;
+  D.nmask = (uint32_t)(n - 1);
+  D.s = (uint32_t)(p & (n - 1));
+  D.q = q;
+  D.lr61 = (uint32_t)lr61;
+  D.lr31 = (uint32_t)lr31;
+  D.d61 = (uint32_t)((lr61 * D.s) % 61u);
+  D.d31 = (uint32_t)((lr31 * D.s) % 31u);
+  D.k61 = (uint32_t)((ln + 1) % 61);
+  D.k31 = (uint32_t)((ln + 1) % 31);
+  // The kernels walk widths and weights incrementally; check the walk against
+  // the tables above for every digit.
+  DWalk w = dwalk_at(D, 0);
+  for (uint64_t j = 0; j < n; ++j) {
+    if (j % 4096 == 0) {
+      const DWalk f = dwalk_at(D, (uint32_t)j);
+      if (f.r != w.r || f.v61 != w.v61 || f.v31 != w.v31) {
+        err = "digit walk: restart mismatch";
+        return false;
+      }
+    }
+    if (dwalk_width(D, w) != T.dig[j].width || dwalk_w61(w) != T.dig[j].w61 || dwalk_w31(w) != T.dig[j].w31) {
+      err = "digit walk mismatch at digit " + std::to_string(j);
+      return false;
+    }
+    dwalk_next(D, w);
   }
   return true;
 }

@@ -11,19 +11,28 @@
 
 namespace m2 {
 
-// Device-side view of one test: buffers and the pass plan, passed by value
-// to the launchers.
+// One test as the kernels see it. Tests of the same transform length share a
+// DevPlan (pass structure and twiddle tables) and run in the same launches,
+// one descriptor per test (grid.y on the multi-pass plan, one workgroup per
+// test on the single-workgroup plan).
+struct TestDesc {
+  e61* zd;       // multi-pass plan: digits as int64 pairs, strip-major (h slots)
+  e61* z61;      // spectrum, M61 part (h elements); single-workgroup plan: the digits
+  e31* z31;      // spectrum, M31 part
+  Carry128* G;   // multi-pass plan: carries not yet applied (nseg)
+  DigitGen dgen;
+  int32_t mode;    // 0 = LL, 1 = PRP
+  uint32_t iters;  // single-workgroup plan: iterations to run in this launch
+};
+
+// Plan and tables of one transform length (shared by every test of that length).
 struct DevPlan {
-  e61* z61 = nullptr;  // h elements: spectrum (M61 part); small plan: also the digits
-  e31* z31 = nullptr;  // h elements: spectrum (M31 part)
-  e61* zd = nullptr;   // multi-pass plan: digits as int64 pairs, strip-major (h slots)
-  const e61* tw61 = nullptr;       // step twiddle tables
+  const e61* tw61 = nullptr;  // step twiddle tables
   const e31* tw31 = nullptr;
   const e61* sq61 = nullptr;  // centre twiddles, h/2 entries
   const e31* sq31 = nullptr;
-  Carry128* G = nullptr;  // pending carries, one per outer-pass row segment
   uint32_t h = 0;
-  int ln = 0, L = 0, mode = 1;
+  int ln = 0, L = 0;
   bool small = false;
   Pass outer{};  // small plan: the whole transform
   int logS = 0, logW = 0;
@@ -32,13 +41,11 @@ struct DevPlan {
   int mid_logS = 0, mid_logW = 0;
   Pass centre{};
   uint32_t nseg = 0;  // number of carry segments (length of G)
-  int seg_elems = 0;  // elements per carry segment of the outer pass (= W)
-  DigitGen dgen{};
 };
 
-// Position of element e's digit pair in z61 between iterations: natural order
-// on the small plan, strip-major (strip cg = column / W, then row, then
-// column within the strip) on the multi-pass plan.
+// Position of element e's digit pair between iterations: natural order on the
+// small plan, strip-major (strip cg = column / W, then row, then column
+// within the strip) on the multi-pass plan.
 inline uint64_t digit_slot(const DevPlan& P, uint64_t e) {
   if (P.small) return e;
   const uint64_t S = uint64_t(1) << P.logS, W = uint64_t(1) << P.logW, R = uint64_t(1) << P.outer.A;
@@ -56,30 +63,37 @@ inline uint64_t carry_target_digit(const DevPlan& P, uint64_t g) {
   return 2 * ((r << P.logS) + (cg << P.logW));
 }
 
-hipError_t launch_small(const DevPlan& P, uint32_t iters, hipStream_t st);
-hipError_t launch_iteration(const DevPlan& P, hipStream_t st);
-// One iteration with each kernel's own dispatch timestamps recorded in
-// starts[k]/stops[k] (hipExtLaunchKernelGGL); returns the kernel names.
-hipError_t launch_iteration_events(const DevPlan& P, hipStream_t st, hipEvent_t* starts, hipEvent_t* stops,
-                                   int max_events, int* count, const char** names);
-hipError_t launch_small_events(const DevPlan& P, uint32_t iters, hipStream_t st, hipEvent_t start, hipEvent_t stop);
+// ---- launchers (m2_kernels.hip); `tests` is a device array of ntests descriptors ----
+hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st);
+hipError_t launch_iteration(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st);
+// One iteration with each kernel's dispatch timestamps in starts[k]/stops[k].
+hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
+                                   hipEvent_t* starts, hipEvent_t* stops, int max_events, int* count,
+                                   const char** names);
+hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
+                               hipEvent_t start, hipEvent_t stop);
 
 // ---- host side (m2_host.cpp) ----
 int transform_log2(uint64_t p);
 
-struct HostTables {
-  std::vector<DigitInfo> dig;      // n
-  std::vector<uint32_t> bitpos;    // n + 1: ceil(p j / n)
+struct ClassTables {
   std::vector<e61> tw61;
   std::vector<e31> tw31;
   std::vector<e61> sq61;  // h/2
   std::vector<e31> sq31;
 };
+struct TestTables {
+  std::vector<DigitInfo> dig;    // n
+  std::vector<uint32_t> bitpos;  // n + 1: ceil(p j / n)
+};
 
-// Builds the pass plan (filling step twiddle offsets) and all tables.
-// ls_cap (the -LS option): 0 = automatic; otherwise the centre block is the
-// largest power of two <= ls_cap (at least 4) and the multi-pass plan is used.
-bool build_plan(uint32_t p, int mode, uint32_t ls_cap, DevPlan& P, HostTables& T, std::string& err);
+// Plan and twiddle tables for transform length 2^ln. ls_cap (the -LS option):
+// 0 = automatic; otherwise the centre block is the largest power of two
+// <= ls_cap (at least 4) and the multi-pass plan is used.
+bool build_class_plan(int ln, uint32_t ls_cap, DevPlan& P, ClassTables& T, std::string& err);
+// Digit tables and walk constants of exponent p on plan P (checks the walk
+// against the tables for every digit, and the carry bound of the plan).
+bool build_test_tables(uint32_t p, const DevPlan& P, TestTables& T, DigitGen& D, std::string& err);
 std::string describe_plan(const DevPlan& P);
 
 // Canonical residue (little-endian, ceil(p/8) bytes) of the state given by

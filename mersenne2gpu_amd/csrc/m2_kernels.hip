@@ -349,10 +349,9 @@ __device__ __forceinline__ void stage_range(e61* S61, e31* S31, const e61* __res
 // ---------------------------------------------------------------------------
 // Small plan: one workgroup holds the whole residue (h <= 2^SMALL_LOG_H) and
 // runs `iters` iterations without leaving LDS. Digits are in natural order.
-__global__ void __launch_bounds__(256) k_small(e61* __restrict__ z61, const e61* __restrict__ tw61,
+__global__ void __launch_bounds__(256) k_small(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                                const e31* __restrict__ tw31, const e61* __restrict__ sq61,
-                                               const e31* __restrict__ sq31, Pass ps, DigitGen D, int mode,
-                                               uint32_t iters) {
+                                               const e31* __restrict__ sq31, Pass ps) {
   __shared__ e61 L61[padded(1 << SMALL_LOG_H)];
   __shared__ e31 L31[padded(1 << SMALL_LOG_H)];
   __shared__ e61 T61[1 << SMALL_LOG_H];
@@ -360,6 +359,9 @@ __global__ void __launch_bounds__(256) k_small(e61* __restrict__ z61, const e61*
   __shared__ e61 S61[1 << (SMALL_LOG_H - 1)];
   __shared__ e31 S31[1 << (SMALL_LOG_H - 1)];
   __shared__ Carry128 gs[(1 << SMALL_LOG_H) / SEG_ELEMS];
+  const TestDesc td = tests[blockIdx.x];
+  e61* __restrict__ z61 = td.z61;
+  const DigitGen D = td.dgen;
   const int logH = ps.A;
   const int h = 1 << logH;
   const int nseg = h / SEG_ELEMS;
@@ -368,14 +370,14 @@ __global__ void __launch_bounds__(256) k_small(e61* __restrict__ z61, const e61*
   stage_range(S61, S31, sq61, sq31, 0, h >> 1);
   for (int i = threadIdx.x; i < h; i += blockDim.x) L61[PADX(i)] = z61[i];
   __syncthreads();
-  for (uint32_t it = 0; it < iters; ++it) {
+  for (uint32_t it = 0; it < td.iters; ++it) {
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_weight(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D);
     __syncthreads();
     lds_pass<true>(L61, L31, 0, 1, ps, T61, T31);
     square_self(L61, L31, logH, 1, S61, S31);
     lds_pass<false>(L61, L31, 0, 1, ps, T61, T31);
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
-      const i128 c0 = (mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
+      const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
       store_carry(gs, s, seg_normalise(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D, c0));
     }
     __syncthreads();
@@ -401,15 +403,18 @@ __device__ __forceinline__ uint32_t prev_strip_carry(uint32_t cg, uint32_t r, ui
 
 constexpr int OUTER_TW = PASS_MAX_ELEMS / 2;  // R - 1 twiddles per block, R <= 512 here
 
-__global__ void __launch_bounds__(256) k_outer_fwd(const e61* __restrict__ zd, e61* __restrict__ z61,
-                                                   e31* __restrict__ z31, const e61* __restrict__ tw61,
-                                                   const e31* __restrict__ tw31,
-                                                   const Carry128* __restrict__ G, Pass ps, int logS, int logW,
-                                                   DigitGen D) {
+__global__ void __launch_bounds__(256) k_outer_fwd(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+                                                   const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
+  const TestDesc td = tests[blockIdx.y];
+  const e61* __restrict__ zd = td.zd;
+  e61* __restrict__ z61 = td.z61;
+  e31* __restrict__ z31 = td.z31;
+  const Carry128* __restrict__ G = td.G;
+  const DigitGen D = td.dgen;
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t cg = blockIdx.x, NS = (1u << logS) >> logW;
   const uint32_t col0 = cg << logW;
@@ -437,16 +442,20 @@ __global__ void __launch_bounds__(256) k_outer_fwd(const e61* __restrict__ zd, e
   }
 }
 
-__global__ void __launch_bounds__(256) k_outer_inv(const e61* __restrict__ z61, const e31* __restrict__ z31,
-                                                   e61* __restrict__ zd, const e61* __restrict__ tw61,
-                                                   const e31* __restrict__ tw31,
-                                                   Carry128* __restrict__ G, Pass ps, int logS, int logW, DigitGen D,
-                                                   int mode) {
+__global__ void __launch_bounds__(256) k_outer_inv(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+                                                   const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
   __shared__ Carry128 gs[PASS_MAX_ELEMS / SEG_ELEMS];
+  const TestDesc td = tests[blockIdx.y];
+  const e61* __restrict__ z61 = td.z61;
+  const e31* __restrict__ z31 = td.z31;
+  e61* __restrict__ zd = td.zd;
+  Carry128* __restrict__ G = td.G;
+  const DigitGen D = td.dgen;
+  const int mode = td.mode;
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t cg = blockIdx.x;
   const uint32_t col0 = cg << logW;
@@ -483,13 +492,15 @@ __global__ void __launch_bounds__(256) k_outer_inv(const e61* __restrict__ z61, 
 // Middle pass: levels [ell0, ell0 + A) with no carry work. Level-ell0 block
 // j (of R * S elements) is split into S/W column groups.
 template <bool FWD>
-__global__ void __launch_bounds__(256) k_mid(e61* __restrict__ z61, e31* __restrict__ z31,
-                                             const e61* __restrict__ tw61, const e31* __restrict__ tw31, Pass ps,
-                                             int logS, int logW) {
+__global__ void __launch_bounds__(256) k_mid(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+                                             const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
+  const TestDesc td = tests[blockIdx.y];
+  e61* __restrict__ z61 = td.z61;
+  e31* __restrict__ z31 = td.z31;
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW;
   const uint32_t jb = blockIdx.x / NS, cg = blockIdx.x - jb * NS;
@@ -514,15 +525,18 @@ __global__ void __launch_bounds__(256) k_mid(e61* __restrict__ z61, e31* __restr
 // C = 2^logC elements that the real-data pairing links; workgroup 0 takes
 // blocks 0 and 1, which pair with themselves. Forward levels [ps.ell0, L),
 // square, inverse. There are h / (2C) workgroups.
-__global__ void __launch_bounds__(256) k_centre(e61* __restrict__ z61, e31* __restrict__ z31,
-                                                const e61* __restrict__ tw61, const e31* __restrict__ tw31,
-                                                const e61* __restrict__ sq61, const e31* __restrict__ sq31, Pass ps) {
+__global__ void __launch_bounds__(256) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+                                                const e31* __restrict__ tw31, const e61* __restrict__ sq61,
+                                                const e31* __restrict__ sq31, Pass ps) {
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
   __shared__ e61 T61[2 << CENTRE_LOG_C];
   __shared__ e31 T31[2 << CENTRE_LOG_C];
   __shared__ e61 S61[1 << CENTRE_LOG_C];
   __shared__ e31 S31[1 << CENTRE_LOG_C];
+  const TestDesc td = tests[blockIdx.y];
+  e61* __restrict__ z61 = td.z61;
+  e31* __restrict__ z31 = td.z31;
   const int logC = ps.A, C = 1 << logC;
   const uint32_t g = blockIdx.x;
   uint32_t B = 0, B2 = 1;
@@ -557,48 +571,48 @@ __global__ void __launch_bounds__(256) k_centre(e61* __restrict__ z61, e31* __re
 // ---------------------------------------------------------------------------
 // Host-side launchers, used by the engine.
 
-hipError_t launch_small(const DevPlan& P, uint32_t iters, hipStream_t st) {
-  hipLaunchKernelGGL(k_small, dim3(1), dim3(256), 0, st, P.z61, P.tw61, P.tw31, P.sq61, P.sq31, P.outer, P.dgen,
-                     P.mode, iters);
+hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
+  hipLaunchKernelGGL(k_small, dim3(ntests), dim3(256), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
   return hipGetLastError();
 }
 
 namespace {
-// Launches the kernels of one iteration through launch(name, kernel, grid, args...).
+// Launches the kernels of one iteration of every test in `tests` through
+// launch(name, kernel, grid, args...); grid.y indexes the tests.
 template <class F>
-hipError_t iteration(const DevPlan& P, F&& launch) {
+hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& launch) {
   hipError_t e;
   const uint32_t ngroups_outer = (1u << P.logS) >> P.logW;
-  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ngroups_outer), P.zd, P.z61, P.z31, P.tw61, P.tw31,
-                  (const Carry128*)P.G, P.outer, P.logS, P.logW, P.dgen)) != hipSuccess)
+  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ngroups_outer, nt), tests, P.tw61, P.tw31, P.outer, P.logS,
+                  P.logW)) != hipSuccess)
     return e;
   const uint32_t nmid = P.has_mid ? (1u << P.outer.A) * ((1u << P.mid_logS) >> P.mid_logW) : 0;
-  if (P.has_mid && (e = launch("k_mid_fwd", k_mid<true>, dim3(nmid), P.z61, P.z31, P.tw61, P.tw31, P.mid,
-                               P.mid_logS, P.mid_logW)) != hipSuccess)
+  if (P.has_mid && (e = launch("k_mid_fwd", k_mid<true>, dim3(nmid, nt), tests, P.tw61, P.tw31, P.mid, P.mid_logS,
+                               P.mid_logW)) != hipSuccess)
     return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
-  if ((e = launch("k_centre", k_centre, dim3(ncentre), P.z61, P.z31, P.tw61, P.tw31, P.sq61, P.sq31, P.centre)) !=
+  if ((e = launch("k_centre", k_centre, dim3(ncentre, nt), tests, P.tw61, P.tw31, P.sq61, P.sq31, P.centre)) !=
       hipSuccess)
     return e;
-  if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid), P.z61, P.z31, P.tw61, P.tw31, P.mid,
-                               P.mid_logS, P.mid_logW)) != hipSuccess)
+  if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid, nt), tests, P.tw61, P.tw31, P.mid, P.mid_logS,
+                               P.mid_logW)) != hipSuccess)
     return e;
-  return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer), (const e61*)P.z61, (const e31*)P.z31, P.zd, P.tw61,
-                P.tw31, P.G, P.outer, P.logS, P.logW, P.dgen, P.mode);
+  return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer, nt), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
 }
 }  // namespace
 
-hipError_t launch_iteration(const DevPlan& P, hipStream_t st) {
-  return iteration(P, [&](const char*, auto kernel, dim3 grid, auto... args) -> hipError_t {
+hipError_t launch_iteration(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
+  return iteration(P, tests, ntests, [&](const char*, auto kernel, dim3 grid, auto... args) -> hipError_t {
     hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, args...);
     return hipGetLastError();
   });
 }
 
-hipError_t launch_iteration_events(const DevPlan& P, hipStream_t st, hipEvent_t* starts, hipEvent_t* stops,
-                                   int max_events, int* count, const char** names) {
+hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
+                                   hipEvent_t* starts, hipEvent_t* stops, int max_events, int* count,
+                                   const char** names) {
   int k = 0;
-  hipError_t e = iteration(P, [&](const char* name, auto kernel, dim3 grid, auto... args) -> hipError_t {
+  hipError_t e = iteration(P, tests, ntests, [&](const char* name, auto kernel, dim3 grid, auto... args) -> hipError_t {
     if (k >= max_events) return hipErrorInvalidValue;
     names[k] = name;
     hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, st, starts[k], stops[k], 0, args...);
@@ -609,9 +623,10 @@ hipError_t launch_iteration_events(const DevPlan& P, hipStream_t st, hipEvent_t*
   return e;
 }
 
-hipError_t launch_small_events(const DevPlan& P, uint32_t iters, hipStream_t st, hipEvent_t start, hipEvent_t stop) {
-  hipExtLaunchKernelGGL(k_small, dim3(1), dim3(256), 0, st, start, stop, 0, P.z61, P.tw61, P.tw31, P.sq61, P.sq31,
-                        P.outer, P.dgen, P.mode, iters);
+hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
+                               hipEvent_t start, hipEvent_t stop) {
+  hipExtLaunchKernelGGL(k_small, dim3(ntests), dim3(256), 0, st, start, stop, 0, tests, P.tw61, P.tw31, P.sq61,
+                        P.sq31, P.outer);
   return hipGetLastError();
 }
 

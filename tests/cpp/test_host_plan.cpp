@@ -29,9 +29,15 @@ int main() {
   for (uint32_t p : ps) {
     for (uint32_t cap : caps) {
       DevPlan P;
-      HostTables T;
+      ClassTables CT;
+      TestTables T;
+      DigitGen D;
       std::string err;
-      const bool ok = build_plan(p, 1, cap, P, T, err);
+      const bool okc = build_class_plan(transform_log2(p), cap, P, CT, err);
+      CHECK(okc, "p=%u cap=%u class plan: %s", p, cap, err.c_str());
+      if (!okc) continue;
+      CHECK(CT.sq61.size() == P.h / 2 && CT.sq31.size() == P.h / 2, "p=%u centre table size", p);
+      const bool ok = build_test_tables(p, P, T, D, err);
       if (!ok) {
         // only the multi-pass carry bound (q >= 12) may refuse a plan
         CHECK(err.find("12 bits") != std::string::npos, "p=%u cap=%u: %s", p, cap, err.c_str());

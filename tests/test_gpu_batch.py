@@ -1,0 +1,160 @@
+"""Batches and the server: each test's result must be the one it gets alone.
+
+Expected values are the committed exact fixtures (tests/golden), which the
+single-test parity suite already checks the engine against one exponent at a
+time. Bit-exact Res64 and residue hashes, no tolerance.
+"""
+import hashlib
+
+import pytest
+
+import mersenne2gpu_amd as m2
+from conftest import load_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _sweep_cases():
+    return [c for c in load_fixture("sweep.json")["cases"] if c["p"] > 3]
+
+
+def test_batch_sweep_mixed_lengths_checkpoints(gpu_ctx):
+    """Every transform length from 2^4 to 2^17, LL and PRP, in one batch;
+    advanced 1, 4, 45, 150 iterations at a time and checked at each stop."""
+    cases = _sweep_cases()
+    print("batch of %d sweep cases" % len(cases), flush=True)
+    with m2.Batch(gpu_ctx, [c["p"] for c in cases], [c["mode"] for c in cases]) as b:
+        done = 0
+        for k in (1, 5, 50, 200):
+            b.run(k - done)
+            done = k
+            for i, c in enumerate(cases):
+                assert b.iteration(i) == k
+                assert b.res64(i) == c["checkpoints"][str(k)]["res64"], (c["p"], c["mode"], k)
+
+
+def test_batch_uneven_stops(gpu_ctx):
+    """Tests stop at their own iteration counts inside one batch (the active
+    set of a launch shrinks as tests finish)."""
+    cases = _sweep_cases()
+    stops = [(1, 5, 50, 200)[i % 4] for i in range(len(cases))]
+    with m2.Batch(gpu_ctx, [c["p"] for c in cases], [c["mode"] for c in cases], iters=stops) as b:
+        b.run(0)
+        b.sync()
+        for i, c in enumerate(cases):
+            assert b.iteration(i) == stops[i]
+            assert b.res64(i) == c["checkpoints"][str(stops[i])]["res64"], (c["p"], c["mode"], stops[i])
+
+
+def test_batch_c4_512_tests(gpu_ctx):
+    """The 512-test batch workload (log-uniform n = 2^8..2^16, PRP): Res64 and
+    residue hash at iteration 2000 for all, and the full test (verdict and
+    final residue) for every case with p <= 60000."""
+    cases = load_fixture("c4.json")["cases"]
+    stops = [c["final"] if c["p"] <= 60000 else 2000 for c in cases]
+    print("c4 batch: %d tests, %d run to completion" % (len(cases), sum(s != 2000 for s in stops)), flush=True)
+    with m2.Batch(gpu_ctx, [c["p"] for c in cases], [c["mode"] for c in cases], iters=stops) as b:
+        b.run(2000)
+        for i, c in enumerate(cases):
+            want = c["checkpoints"]["2000"]
+            assert b.res64(i) == want["res64"], (c["p"], 2000)
+            assert hashlib.sha256(b.residue(i)).hexdigest() == want["sha256"], (c["p"], 2000)
+        b.run(0)
+        b.sync()
+        for i, c in enumerate(cases):
+            if stops[i] == 2000:
+                continue
+            want = c["checkpoints"][str(stops[i])]
+            assert b.iteration(i) == stops[i]
+            assert b.res64(i) == want["res64"], (c["p"], stops[i])
+            assert hashlib.sha256(b.residue(i)).hexdigest() == want["sha256"], (c["p"], stops[i])
+            # a composite's PRP residue is not 9 unless it is a base-3 probable prime
+            assert b.is_prime(i) == (want["res64"] == "0000000000000009")
+
+
+def test_batch_full_tests_verdicts(gpu_ctx):
+    """Known primes and composites run to their verdict inside one batch."""
+    mersenne_prime_exps = {3, 5, 7, 13, 17, 19, 31, 61, 89, 107, 127, 521, 607, 1279, 2203, 2281, 9941}
+    cases = [c for c in load_fixture("kat.json")["cases"] if c["p"] > 3]
+    with m2.Batch(gpu_ctx, [c["p"] for c in cases], [c["mode"] for c in cases]) as b:
+        b.run(0)
+        for i, c in enumerate(cases):
+            assert b.iteration(i) == c["final"]
+            assert b.res64(i) == c["checkpoints"][str(c["final"])]["res64"], (c["p"], c["mode"])
+            assert b.is_prime(i) == (c["p"] in mersenne_prime_exps), (c["p"], c["mode"])
+
+
+def test_batch_multipass_plan(gpu_ctx):
+    """Several tests sharing the multi-pass launches (grid row per test)."""
+    cases = [c for c in _sweep_cases() if 6 <= c["ln"] <= 17 and (c["p"] >> c["ln"]) >= 12]
+    with m2.Batch(gpu_ctx, [c["p"] for c in cases], [c["mode"] for c in cases], local_stage_cap=16) as b:
+        done = 0
+        for k in (5, 200):
+            b.run(k - done)
+            done = k
+            for i, c in enumerate(cases):
+                assert b.res64(i) == c["checkpoints"][str(k)]["res64"], (c["p"], c["mode"], k)
+
+
+def test_batch_rejects_bad_input(gpu_ctx):
+    with pytest.raises(m2.M2Error):
+        m2.Batch(gpu_ctx, [127, 2], ["prp", "prp"])
+    with pytest.raises(ValueError):
+        m2.Batch(gpu_ctx, [127], ["xyz"])
+    with m2.Batch(gpu_ctx, [127], ["prp"]) as b:
+        with pytest.raises(m2.M2Error):
+            b.res64(1)
+
+
+def test_server_results_match_alone(gpu_ctx):
+    """Requests of mixed lengths through a small memory budget: admission
+    waits for memory, every completion equals the test's result alone."""
+    sweep = {(c["p"], c["mode"]): c for c in _sweep_cases()}
+    reqs = []
+    for (p, mode), c in sorted(sweep.items()):
+        if c["ln"] >= 6:
+            reqs.append((p, mode, 200 if c["ln"] % 2 else 50, c))
+    c4 = load_fixture("c4.json")["cases"]
+    for c in c4[:40]:
+        k = c["final"] if c["p"] <= 60000 else 2000
+        reqs.append((c["p"], c["mode"], k, c))
+    big = max(m2.footprint(p) for p, _, _, _ in reqs)
+    budget = 3 * big
+    print("server: %d requests, budget %d bytes" % (len(reqs), budget), flush=True)
+    with m2.Server(gpu_ctx, hbm_budget_bytes=budget, slice_iters=1000) as s:
+        ids = {}
+        for p, mode, k, c in reqs:
+            ids[s.submit(p, mode, k)] = (p, mode, k, c)
+        got = {}
+        steps = 0
+        while True:
+            busy = s.step()
+            steps += 1
+            st = s.stats()
+            assert st["bytes_in_use"] <= budget or st["active"] == 1
+            for rid, r64, verdict in s.poll():
+                got[rid] = (r64, verdict)
+            if not busy:
+                break
+            assert steps < 10000
+        for rid, r64, verdict in s.poll():
+            got[rid] = (r64, verdict)
+        assert set(got) == set(ids)
+        for rid, (p, mode, k, c) in ids.items():
+            assert got[rid][0] == c["checkpoints"][str(k)]["res64"], (p, mode, k)
+        st = s.stats()
+        assert st == {"queued": 0, "active": 0, "bytes_in_use": 0}
+
+
+def test_server_one_request_larger_than_budget_runs_alone(gpu_ctx):
+    c = [c for c in _sweep_cases() if c["ln"] == 12 and c["mode"] == "prp"][0]
+    with m2.Server(gpu_ctx, hbm_budget_bytes=1, slice_iters=7) as s:
+        a = s.submit(c["p"], "prp", 50)
+        b = s.submit(c["p"], "prp", 200)
+        out = {}
+        while s.step():
+            assert s.stats()["active"] <= 1
+            out.update({rid: r64 for rid, r64, _ in s.poll()})
+        out.update({rid: r64 for rid, r64, _ in s.poll()})
+        assert out[a] == c["checkpoints"]["50"]["res64"]
+        assert out[b] == c["checkpoints"]["200"]["res64"]
