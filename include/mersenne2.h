@@ -43,12 +43,17 @@ typedef struct m2_batch m2_batch;
 /* Tuning options; zero fields mean "automatic". The reference's -W, -CW, -B
  * and -LS flags (mersenne2_gpu.cpp:3660-3681) map onto these. */
 typedef struct m2_opts {
-  uint32_t wg;              /* -W: work-group size hint */
-  uint32_t carry_wg;        /* -CW: carry work-group size hint */
-  uint32_t carry_pairs;     /* -B: carry block size hint in element pairs */
-  uint32_t local_stage_cap; /* -LS: cap on LDS-staged transform points */
-  uint32_t graph_iters;     /* iterations per captured hipGraph (0: automatic) */
-  uint32_t reserved[3];
+  uint32_t wg;              /* -W: work-group size hint (the kernels use 256) */
+  uint32_t carry_wg;        /* -CW: carry work-group size hint (unused: carries run inside the passes) */
+  uint32_t carry_pairs;     /* -B: digit pairs per carry block (a strip row of the outer pass),
+                               rounded down to a power of two and clamped to the plan */
+  uint32_t local_stage_cap; /* -LS: cap on the points of the LDS-resident centre block; a nonzero
+                               value also selects the multi-pass plan on short transforms */
+  uint32_t graph_iters;     /* iterations per captured hipGraph (0: 16) */
+  uint32_t max_radix;       /* 8 (0: default) or 4: largest radix of the transform steps */
+  uint32_t autotune;        /* batches and servers: choose the four fields above per transform
+                               length by timing candidates (m2_autotune) */
+  uint32_t reserved;
 } m2_opts;
 
 /* Message of the calling thread's last failure ("" if none). */
@@ -108,6 +113,16 @@ int m2_test_kernel_times(m2_test *t, uint64_t iters, double *ms, int max_kernels
                          size_t names_len);
 /* Human-readable plan (the reference's --debug-plan, mersenne2_gpu.cpp:4767-4776). */
 int m2_test_plan(m2_test *t, char *buf, size_t len);
+
+/* Autotuner: for the transform length of p and a batch of `ntests` tests,
+ * time the candidate plans (single workgroup or multi-pass; centre size, carry
+ * block width, radix, then graph length) and write the fastest into *out
+ * (local_stage_cap, carry_pairs, max_radix, graph_iters). Results are cached
+ * per context by length and batch-size bucket. Every candidate computes the
+ * same residues. `report` (optional) receives one line per candidate with its
+ * device time per iteration. The reference's counterpart is its device-name
+ * profile table and planner (mersenne2_gpu.cpp:3515-3616, :4034-4110). */
+int m2_autotune(m2_ctx *ctx, uint64_t p, uint32_t ntests, m2_opts *out, char *report, size_t report_len);
 
 /*
  * Batches: many tests, of any mix of exponents, resident on one device and

@@ -48,6 +48,7 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "m2_test_kernel_times": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double), _c.c_int, _c.POINTER(_c.c_int),
                                         _c.c_char_p, _c.c_size_t]),
     "m2_test_plan": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
+    "m2_autotune": (_c.c_int, [_P, _c.c_uint64, _c.c_uint32, _P, _c.c_char_p, _c.c_size_t]),
     "m2_batch_create": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
                                    _c.POINTER(_c.c_uint64), _P, _c.POINTER(_P)]),
     "m2_batch_destroy": (_c.c_int, [_P]),
@@ -151,21 +152,39 @@ class Context:
         return Test(self, p, mode, graph_iters)
 
 
+def autotune(ctx: "Context", p: int, ntests: int = 1) -> Tuple[Dict[str, int], str]:
+    """Plan options the autotuner picks for p's transform length and a batch of
+    ntests tests, and its per-candidate report (m2_autotune)."""
+    o = _Opts()
+    buf = ctypes.create_string_buffer(8192)
+    _chk(lib().m2_autotune(ctx.h, int(p), int(ntests), ctypes.byref(o), buf, 8192), "m2_autotune")
+    return o.as_dict(), buf.value.decode()
+
+
 class _Opts(ctypes.Structure):
     _fields_ = [("wg", ctypes.c_uint32), ("carry_wg", ctypes.c_uint32), ("carry_pairs", ctypes.c_uint32),
                 ("local_stage_cap", ctypes.c_uint32), ("graph_iters", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 3)]
+                ("max_radix", ctypes.c_uint32), ("autotune", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self) -> Dict[str, int]:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+def make_opts(**kw) -> "_Opts":
+    o = _Opts()
+    for k, v in kw.items():
+        setattr(o, k, int(v or 0))
+    return o
 
 
 class Test:
     """One LL or PRP test resident on the device (mirror of m2_test_*)."""
 
-    def __init__(self, ctx: Context, p: int, mode: str = "prp", graph_iters: int = 0):
+    def __init__(self, ctx: Context, p: int, mode: str = "prp", graph_iters: int = 0, **opts):
         self.p = int(p)
         self.mode = mode
-        m = MODE_PRP if mode == "prp" else MODE_LL
-        opts = _Opts()
-        opts.graph_iters = graph_iters
+        m = _mode(mode)
+        opts = make_opts(graph_iters=graph_iters, **opts)
         h = ctypes.c_void_p()
         _chk(lib().m2_test_create(ctx.h, self.p, m, ctypes.byref(opts), ctypes.byref(h)), "m2_test_create")
         self.h = h
@@ -264,7 +283,7 @@ class Batch:
     full test (p PRP iterations, p - 2 LL steps)."""
 
     def __init__(self, ctx: Context, ps: Sequence[int], modes, iters: Optional[Sequence[int]] = None,
-                 local_stage_cap: int = 0):
+                 local_stage_cap: int = 0, **opts):
         n = len(ps)
         if isinstance(modes, str):
             modes = [modes] * n
@@ -275,8 +294,7 @@ class Batch:
         It = None
         if iters is not None:
             It = (ctypes.c_uint64 * max(n, 1))(*[int(k or 0) for k in iters])
-        opts = _Opts()
-        opts.local_stage_cap = local_stage_cap
+        opts = make_opts(local_stage_cap=local_stage_cap, **opts)
         h = ctypes.c_void_p()
         _chk(lib().m2_batch_create(ctx.h, n, P, M, It, ctypes.byref(opts), ctypes.byref(h)), "m2_batch_create")
         self.h = h
@@ -332,9 +350,9 @@ class Batch:
 class Server:
     """Request queue over resident tests (mirror of m2_server_*)."""
 
-    def __init__(self, ctx: Context, hbm_budget_bytes: int = 0, slice_iters: int = 0, local_stage_cap: int = 0):
-        opts = _Opts()
-        opts.local_stage_cap = local_stage_cap
+    def __init__(self, ctx: Context, hbm_budget_bytes: int = 0, slice_iters: int = 0, local_stage_cap: int = 0,
+                 **opts):
+        opts = make_opts(local_stage_cap=local_stage_cap, **opts)
         h = ctypes.c_void_p()
         _chk(lib().m2_server_create(ctx.h, int(hbm_budget_bytes), int(slice_iters), ctypes.byref(opts),
                                     ctypes.byref(h)), "m2_server_create")

@@ -11,11 +11,13 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "m2_internal.h"
@@ -42,9 +44,19 @@ constexpr uint32_t GRAPH_ITERS = 16;    // iterations per captured graph (multi-
 
 uint64_t full_iters(uint32_t p, int mode) { return mode == M2_MODE_LL ? (p >= 2 ? p - 2 : 0) : p; }
 
+PlanOpts plan_opts(const m2_opts* o) {
+  PlanOpts r;
+  if (o) {
+    r.ls_cap = o->local_stage_cap;
+    r.carry_pairs = o->carry_pairs;
+    r.max_radix = o->max_radix;
+  }
+  return r;
+}
+
 struct Class {
   int ln = 0;
-  uint32_t ls_cap = 0;
+  PlanOpts popts;
   DevPlan P;
   std::vector<void*> allocs;
   uint64_t bytes = 0;
@@ -89,14 +101,16 @@ struct m2_ctx {
   int device = 0;
   hipDeviceProp_t prop{};
   std::mutex mu;
-  std::map<std::pair<int, uint32_t>, std::unique_ptr<Class>> classes;
+  std::map<std::pair<int, PlanOpts>, std::unique_ptr<Class>> classes;
+  // autotuned choices per (transform length, batch-size bucket)
+  std::map<std::pair<int, uint32_t>, m2_opts> tuned;
 };
 
 namespace {
 
-int get_class(m2_ctx* ctx, int ln, uint32_t ls_cap, Class** out) {
+int get_class(m2_ctx* ctx, int ln, const PlanOpts& po, Class** out) {
   std::lock_guard<std::mutex> lk(ctx->mu);
-  auto key = std::make_pair(ln, ls_cap);
+  auto key = std::make_pair(ln, po);
   auto it = ctx->classes.find(key);
   if (it != ctx->classes.end()) {
     *out = it->second.get();
@@ -104,10 +118,10 @@ int get_class(m2_ctx* ctx, int ln, uint32_t ls_cap, Class** out) {
   }
   std::unique_ptr<Class> c(new Class);
   c->ln = ln;
-  c->ls_cap = ls_cap;
+  c->popts = po;
   ClassTables T;
   std::string err;
-  if (!build_class_plan(ln, ls_cap, c->P, T, err)) return fail(M2_E_RANGE, err);
+  if (!build_class_plan(ln, po, c->P, T, err)) return fail(M2_E_RANGE, err);
   void* d;
   int rc;
   auto bail = [&](int code) {
@@ -148,13 +162,13 @@ int upload_digits(TestObj* t, const int64_t* digits, hipStream_t st) {
   return M2_OK;
 }
 
-int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, uint32_t ls_cap, hipStream_t st, TestObj** out) {
+int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, const PlanOpts& po, hipStream_t st, TestObj** out) {
   if (mode != M2_MODE_LL && mode != M2_MODE_PRP) return fail(M2_E_INVAL, "mode must be LL (0) or PRP (1)");
   if (p < 3 || p > 0xFFFFFFFFull) return fail(M2_E_RANGE, "exponent out of range (3 <= p < 2^32)");
   const int ln = transform_log2(p);
   if (ln > 27) return fail(M2_E_RANGE, "exponent too large: transform length above 2^27");
   Class* cls = nullptr;
-  int rc = get_class(ctx, ln, ls_cap, &cls);
+  int rc = get_class(ctx, ln, po, &cls);
   if (rc) return rc;
   std::unique_ptr<TestObj> t(new TestObj);
   t->cls = cls;
@@ -245,6 +259,7 @@ struct Group {
   static constexpr uint32_t MAX_GRID_ROWS = 65535;  // tests per multi-pass launch (grid.y)
   Class* cls = nullptr;
   hipStream_t stream = nullptr;
+  uint32_t graph_iters = GRAPH_ITERS;  // used when advance() is given 0
   struct Round {
     TestDesc* d = nullptr;
     uint32_t n = 0;
@@ -362,7 +377,7 @@ struct Group {
   int prepare(const std::vector<TestObj*>& tests, uint32_t graph_iters) {
     if (cls->P.small || tests.size() > MAX_GRID_ROWS) return M2_OK;
     Round* r = nullptr;
-    return get_round(tests, graph_iters ? graph_iters : GRAPH_ITERS, &r);
+    return get_round(tests, graph_iters ? graph_iters : this->graph_iters, &r);
   }
   // Advance every test towards its target, at most `limit` iterations each
   // (0: no limit), enqueuing launches without waiting.
@@ -370,7 +385,7 @@ struct Group {
     tests.erase(std::remove_if(tests.begin(), tests.end(), [](TestObj* t) { return t->iter >= t->target; }),
                 tests.end());
     if (tests.empty()) return M2_OK;
-    const uint32_t G = graph_iters ? graph_iters : GRAPH_ITERS;
+    const uint32_t G = graph_iters ? graph_iters : this->graph_iters;
     std::vector<uint64_t> stop(tests.size());
     for (size_t i = 0; i < tests.size(); ++i)
       stop[i] = limit ? std::min(tests[i]->target, tests[i]->iter + limit) : tests[i]->target;
@@ -502,10 +517,10 @@ int m2_test_create(m2_ctx* ctx, uint64_t p, int mode, const m2_opts* opts, m2_te
   const int ln = (p >= 3 && p <= 0xFFFFFFFFull) ? transform_log2(p) : 0;
   if (ln < 3 || ln > 27) return fail(M2_E_RANGE, "exponent out of range (3 <= p < 2^32, n <= 2^27)");
   Class* cls = nullptr;
-  int rc = get_class(ctx, ln, opts ? opts->local_stage_cap : 0u, &cls);
+  int rc = get_class(ctx, ln, plan_opts(opts), &cls);
   if (rc) return rc;
   if ((rc = t->group.create(cls))) return rc;
-  if ((rc = test_obj_create(ctx, p, mode, opts ? opts->local_stage_cap : 0u, t->group.stream, &t->obj))) {
+  if ((rc = test_obj_create(ctx, p, mode, plan_opts(opts), t->group.stream, &t->obj))) {
     t->group.destroy();
     return rc;
   }
@@ -670,6 +685,193 @@ int m2_test_plan(m2_test* t, char* buf, size_t len) {
   return M2_OK;
 }
 
+}  // extern "C"
+
+// ---- autotuner ----
+//
+// For one transform length and batch size, times each candidate plan (single
+// workgroup or multi-pass with several centre sizes, outer carry-block
+// widths and radices, then the graph length of the best) on `ntests` copies
+// of a test and keeps the fastest. Every candidate computes the same
+// residues; only speed differs.
+
+namespace {
+
+uint32_t tune_bucket(uint32_t n) {
+  uint32_t b = 1;
+  while (b < n && b < 256) b <<= 1;
+  return b;
+}
+
+std::string opts_str(const m2_opts& o) {
+  char buf[128];
+  snprintf(buf, sizeof buf, "LS=%u B=%u radix=%u graph=%u", o.local_stage_cap, o.carry_pairs,
+           o.max_radix ? o.max_radix : 8u, o.graph_iters ? o.graph_iters : GRAPH_ITERS);
+  return buf;
+}
+
+// Device µs per iteration of `ntests` tests of exponent p under plan options o.
+int time_candidate(m2_ctx* ctx, uint64_t p, uint32_t ntests, const m2_opts& o, double* us, bool* created) {
+  const int ln = transform_log2(p);
+  const PlanOpts po = plan_opts(&o);
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *created = ctx->classes.find(std::make_pair(ln, po)) == ctx->classes.end();
+  }
+  Class* cls = nullptr;
+  int rc = get_class(ctx, ln, po, &cls);
+  if (rc) return rc;
+  Group g;
+  if ((rc = g.create(cls))) return rc;
+  g.graph_iters = o.graph_iters ? o.graph_iters : GRAPH_ITERS;
+  std::vector<TestObj*> ts;
+  auto cleanup = [&](int code) {
+    g.destroy();
+    for (TestObj* t : ts) test_obj_destroy(t);
+    return code;
+  };
+  for (uint32_t i = 0; i < ntests; ++i) {
+    TestObj* t = nullptr;
+    if ((rc = test_obj_create(ctx, p, M2_MODE_PRP, po, g.stream, &t))) return cleanup(rc);
+    t->target = ~0ull;
+    ts.push_back(t);
+  }
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return cleanup(fail(M2_E_HIP, "event"));
+  auto run = [&](uint64_t k, float* ms) -> int {
+    int r = g.prepare(ts, 0);
+    if (r) return r;
+    HIPCHK(hipEventRecord(a, g.stream));
+    if ((r = g.advance(ts, k, 0))) return r;
+    HIPCHK(hipEventRecord(b, g.stream));
+    HIPCHK(hipEventSynchronize(b));
+    HIPCHK(hipEventElapsedTime(ms, a, b));
+    return g.sync();
+  };
+  float ms = 0;
+  uint64_t k = 32;
+  rc = run(16, &ms);
+  for (int rep = 0; !rc && rep < 3; ++rep) {
+    if ((rc = run(k, &ms))) break;
+    if (ms >= 4.0f) break;
+    k = std::min<uint64_t>(8192, (uint64_t)(k * std::max(2.0, 8.0 / std::max(ms, 0.01f))));
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (!rc) *us = ms * 1e3 / (double)k;
+  return cleanup(rc);
+}
+
+int autotune(m2_ctx* ctx, uint64_t p, uint32_t ntests, m2_opts* out, std::string* report) {
+  const int ln = transform_log2(p);
+  const int L = ln - 1;
+  const uint32_t bucket = tune_bucket(ntests ? ntests : 1);
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->tuned.find(std::make_pair(ln, bucket));
+    if (it != ctx->tuned.end()) {
+      *out = it->second;
+      if (report) *report += "ln=" + std::to_string(ln) + " (cached): " + opts_str(*out) + "\n";
+      return M2_OK;
+    }
+  }
+  std::vector<m2_opts> cand;
+  auto add = [&](uint32_t ls, uint32_t cp, uint32_t mr) {
+    m2_opts o{};
+    o.local_stage_cap = ls;
+    o.carry_pairs = cp;
+    o.max_radix = mr;
+    cand.push_back(o);
+  };
+  if (L <= SMALL_LOG_H) {
+    add(0, 0, 8);
+    add(0, 0, 4);
+  }
+  for (int cl : {6, 8, 10}) {
+    if (L < cl + 2) continue;
+    if (L <= SMALL_LOG_H && cl < L - 6) continue;  // small lengths: only the shallowest outer passes
+    add(2u << cl, 0, 8);
+    add(2u << cl, 0, 4);
+    add(2u << cl, 16, 8);
+  }
+  const uint32_t nt = std::min<uint32_t>(std::max<uint32_t>(ntests, 1), 256);
+  std::vector<std::pair<int, PlanOpts>> made;
+  double best = 1e300;
+  m2_opts bo{};
+  int rc = M2_OK;
+  auto try_one = [&](const m2_opts& o) -> int {
+    double us = 0;
+    bool created = false;
+    int r = time_candidate(ctx, p, nt, o, &us, &created);
+    if (created) made.push_back(std::make_pair(ln, plan_opts(&o)));
+    if (r == M2_E_RANGE) {  // plan not valid for this exponent: skip
+      if (report) *report += "  " + opts_str(o) + ": not applicable\n";
+      return M2_OK;
+    }
+    if (r) return r;
+    if (report) {
+      char buf[64];
+      snprintf(buf, sizeof buf, ": %.3f us/iter\n", us);
+      *report += "  " + opts_str(o) + buf;
+    }
+    if (us < best) {
+      best = us;
+      bo = o;
+    }
+    return M2_OK;
+  };
+  if (report) *report += "ln=" + std::to_string(ln) + " tests=" + std::to_string(nt) + ":\n";
+  for (const m2_opts& o : cand)
+    if ((rc = try_one(o))) break;
+  const bool multi = !rc && best < 1e300 && !(L <= SMALL_LOG_H && bo.local_stage_cap == 0);
+  if (multi) {
+    for (uint32_t gi : {8u, 32u}) {
+      m2_opts o = bo;
+      o.graph_iters = gi;
+      if ((rc = try_one(o))) break;
+    }
+  }
+  // free the tables of candidates that lost (and were made only for tuning)
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (auto& key : made) {
+      if (key.second < plan_opts(&bo) || plan_opts(&bo) < key.second) {
+        auto it = ctx->classes.find(key);
+        if (it != ctx->classes.end()) {
+          free_all(it->second->allocs);
+          ctx->classes.erase(it);
+        }
+      }
+    }
+  }
+  if (rc) return rc;
+  if (best >= 1e300) return fail(M2_E_RANGE, "no applicable plan");
+  if (report) {
+    char buf[64];
+    snprintf(buf, sizeof buf, " (%.3f us/iter)\n", best);
+    *report += "  best: " + opts_str(bo) + buf;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->tuned[std::make_pair(ln, bucket)] = bo;
+  *out = bo;
+  return M2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int m2_autotune(m2_ctx* ctx, uint64_t p, uint32_t ntests, m2_opts* out, char* report, size_t report_len) {
+  if (!ctx || !out) return fail(M2_E_INVAL, "null pointer");
+  if (p < 3 || p > 0xFFFFFFFFull || transform_log2(p) > 27) return fail(M2_E_RANGE, "exponent out of range");
+  if ((p & 1u) == 0) return fail(M2_E_RANGE, "exponent must be odd");
+  HIPCHK(hipSetDevice(ctx->device));
+  std::string rep;
+  int rc = autotune(ctx, p, ntests, out, report ? &rep : nullptr);
+  if (report && report_len) snprintf(report, report_len, "%s", rep.c_str());
+  return rc;
+}
+
 // ---- batches ----
 
 int m2_batch_create(m2_ctx* ctx, size_t count, const uint64_t* ps, const int* modes, const uint64_t* iters,
@@ -678,7 +880,25 @@ int m2_batch_create(m2_ctx* ctx, size_t count, const uint64_t* ps, const int* mo
   HIPCHK(hipSetDevice(ctx->device));
   std::unique_ptr<m2_batch> b(new m2_batch);
   b->ctx = ctx;
-  const uint32_t ls = opts ? opts->local_stage_cap : 0u;
+  // plan options per transform length: the caller's, or tuned for this batch
+  std::map<int, m2_opts> per_ln;
+  std::map<int, uint32_t> count_ln;
+  for (size_t i = 0; i < count; ++i)
+    if (ps[i] >= 3 && ps[i] <= 0xFFFFFFFFull) ++count_ln[transform_log2(ps[i])];
+  for (auto& kv : count_ln) {
+    m2_opts o{};
+    if (opts) o = *opts;
+    if (opts && opts->autotune && kv.first <= 27) {
+      uint64_t rep_p = 0;
+      for (size_t i = 0; i < count && !rep_p; ++i)
+        if (ps[i] >= 3 && ps[i] <= 0xFFFFFFFFull && transform_log2(ps[i]) == kv.first && (ps[i] & 1)) rep_p = ps[i];
+      if (rep_p) {
+        int rc = autotune(ctx, rep_p, kv.second, &o, nullptr);
+        if (rc) return rc;
+      }
+    }
+    per_ln[kv.first] = o;
+  }
   auto bail = [&](int code) {
     for (auto& kv : b->groups) kv.second.destroy();
     for (TestObj* t : b->tests) test_obj_destroy(t);
@@ -687,17 +907,19 @@ int m2_batch_create(m2_ctx* ctx, size_t count, const uint64_t* ps, const int* mo
   for (size_t i = 0; i < count; ++i) {
     const int ln = (ps[i] >= 3 && ps[i] <= 0xFFFFFFFFull) ? transform_log2(ps[i]) : 0;
     if (ln < 3 || ln > 27) return bail(fail(M2_E_RANGE, "exponent out of range at index " + std::to_string(i)));
+    const m2_opts& o = per_ln[ln];
     Class* cls = nullptr;
-    int rc = get_class(ctx, ln, ls, &cls);
+    int rc = get_class(ctx, ln, plan_opts(&o), &cls);
     if (rc) return bail(rc);
     auto it = b->groups.find(cls);
     if (it == b->groups.end()) {
       Group g;
       if ((rc = g.create(cls))) return bail(rc);
+      if (o.graph_iters) g.graph_iters = o.graph_iters;
       it = b->groups.emplace(cls, g).first;
     }
     TestObj* t = nullptr;
-    if ((rc = test_obj_create(ctx, ps[i], modes[i], ls, it->second.stream, &t))) return bail(rc);
+    if ((rc = test_obj_create(ctx, ps[i], modes[i], plan_opts(&o), it->second.stream, &t))) return bail(rc);
     if (iters && iters[i]) t->target = iters[i];
     b->tests.push_back(t);
   }
@@ -719,9 +941,20 @@ int m2_batch_run(m2_batch* b, uint64_t max_iters) {
   HIPCHK(hipSetDevice(b->ctx->device));
   std::map<Class*, std::vector<TestObj*>> by;
   for (TestObj* t : b->tests) by[t->cls].push_back(t);
-  for (auto& kv : by) {
-    int rc = b->groups[kv.first].advance(kv.second, max_iters, GRAPH_ITERS);
-    if (rc) return rc;
+  // Enqueue in chunks, round-robin over the lengths, so that every length's
+  // stream has work queued early instead of one stream taking the whole run.
+  constexpr uint64_t CHUNK = 8192;
+  uint64_t left = max_iters ? max_iters : ~0ull;
+  while (left) {
+    const uint64_t k = std::min(left, CHUNK);
+    bool any = false;
+    for (auto& kv : by) {
+      for (TestObj* t : kv.second) any |= t->iter < t->target;
+      int rc = b->groups[kv.first].advance(kv.second, k, 0);
+      if (rc) return rc;
+    }
+    if (!any) break;
+    left -= k;
   }
   return M2_OK;
 }
@@ -790,25 +1023,38 @@ int m2_batch_is_prime(m2_batch* b, size_t i, int* verdict) {
 }  // extern "C"
 
 // ---- server ----
+//
+// Continuous batching: each transform length has a lane (a group on its own
+// stream) that runs one slice of its members at a time. A lane whose slice
+// has finished retires its finished tests and starts the next slice with
+// whatever members it has then, including tests admitted meanwhile, while
+// the other lanes keep running.
 
 struct m2_server {
   m2_ctx* ctx = nullptr;
   uint64_t budget = 0, in_use = 0;
   uint32_t slice = 1000;
-  uint32_t ls_cap = 0;
-  std::mutex mu;  // guards `queue` and `done` (submit/poll may come from other threads)
+  m2_opts opts{};
+  hipStream_t setup = nullptr;  // uploads of new tests' initial state
+  std::mutex mu;                // guards `queue` and `done` (submit/poll may come from other threads)
   struct Req {
     uint64_t id, p, iters;
     int mode;
   };
   std::deque<Req> queue;
-  struct Active {
+  struct Member {
     uint64_t id;
     TestObj* t;
     uint64_t bytes;
   };
-  std::vector<Active> active;
-  std::map<Class*, Group> groups;
+  struct Lane {
+    Group g;
+    std::vector<Member> members;
+    hipEvent_t ev = nullptr;
+    bool inflight = false;
+  };
+  std::map<Class*, Lane> lanes;
+  uint64_t nactive = 0;
   struct Done {
     uint64_t id, res64;
     int verdict;
@@ -817,16 +1063,101 @@ struct m2_server {
   uint64_t next_id = 1;
 };
 
+namespace {
+
+int server_admit(m2_server* s, bool* progressed) {
+  for (;;) {
+    m2_server::Req rq;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->queue.empty()) return M2_OK;
+      rq = s->queue.front();
+    }
+    uint64_t fp = 0;
+    m2_test_footprint(rq.p, &fp);
+    if (s->in_use + fp > s->budget && s->nactive > 0) return M2_OK;
+    m2_opts o = s->opts;
+    if (s->opts.autotune) {
+      // tuned for a lane of a few tests of this length
+      int rc = autotune(s->ctx, rq.p, 8, &o, nullptr);
+      if (rc) return rc;
+    }
+    Class* cls = nullptr;
+    int rc = get_class(s->ctx, transform_log2(rq.p), plan_opts(&o), &cls);
+    if (rc) return rc;
+    auto it = s->lanes.find(cls);
+    if (it == s->lanes.end()) {
+      m2_server::Lane lane;
+      if ((rc = lane.g.create(cls))) return rc;
+      if (o.graph_iters) lane.g.graph_iters = o.graph_iters;
+      HIPCHK(hipEventCreateWithFlags(&lane.ev, hipEventDisableTiming));
+      it = s->lanes.emplace(cls, lane).first;
+    }
+    TestObj* t = nullptr;
+    if ((rc = test_obj_create(s->ctx, rq.p, rq.mode, plan_opts(&o), s->setup, &t))) return rc;
+    if (rq.iters) t->target = std::min<uint64_t>(t->target, rq.iters);
+    it->second.members.push_back({rq.id, t, fp});
+    s->in_use += fp;
+    ++s->nactive;
+    *progressed = true;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->queue.pop_front();
+  }
+}
+
+// Lane whose slice is done (or idle): retire finished members, start the next slice.
+int server_service_lane(m2_server* s, m2_server::Lane& lane, bool* progressed) {
+  if (lane.inflight) {
+    hipError_t q = hipEventQuery(lane.ev);
+    if (q == hipErrorNotReady) return M2_OK;
+    if (q != hipSuccess) return fail(M2_E_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
+    lane.inflight = false;
+    *progressed = true;
+  }
+  int rc = lane.g.sync();
+  if (rc) return rc;
+  std::vector<m2_server::Member> keep;
+  for (auto& m : lane.members) {
+    if (m.t->iter < m.t->target) {
+      keep.push_back(m);
+      continue;
+    }
+    std::vector<uint8_t> r;
+    if ((rc = test_obj_residue(m.t, r))) return rc;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->done.push_back({m.id, res64_of(r), verdict_of(m.t, r)});
+    }
+    s->in_use -= m.bytes;
+    --s->nactive;
+    if ((rc = lane.g.forget(m.t))) return rc;
+    test_obj_destroy(m.t);
+    *progressed = true;
+  }
+  lane.members.swap(keep);
+  if (lane.members.empty()) return M2_OK;
+  std::vector<TestObj*> ts;
+  for (auto& m : lane.members) ts.push_back(m.t);
+  if ((rc = lane.g.advance(ts, s->slice, 0))) return rc;
+  HIPCHK(hipEventRecord(lane.ev, lane.g.stream));
+  lane.inflight = true;
+  return M2_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int m2_server_create(m2_ctx* ctx, uint64_t hbm_budget_bytes, uint32_t slice_iters, const m2_opts* opts,
                      m2_server** out) {
   if (!ctx || !out) return fail(M2_E_INVAL, "null pointer");
+  HIPCHK(hipSetDevice(ctx->device));
   std::unique_ptr<m2_server> s(new m2_server);
   s->ctx = ctx;
   s->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(0.7 * (double)ctx->prop.totalGlobalMem);
   s->slice = slice_iters ? slice_iters : 1000;
-  s->ls_cap = opts ? opts->local_stage_cap : 0u;
+  if (opts) s->opts = *opts;
+  HIPCHK(hipStreamCreateWithFlags(&s->setup, hipStreamNonBlocking));
   *out = s.release();
   return M2_OK;
 }
@@ -834,8 +1165,12 @@ int m2_server_create(m2_ctx* ctx, uint64_t hbm_budget_bytes, uint32_t slice_iter
 int m2_server_destroy(m2_server* s) {
   if (!s) return M2_OK;
   (void)hipSetDevice(s->ctx->device);
-  for (auto& kv : s->groups) kv.second.destroy();
-  for (auto& a : s->active) test_obj_destroy(a.t);
+  for (auto& kv : s->lanes) {
+    kv.second.g.destroy();
+    for (auto& m : kv.second.members) test_obj_destroy(m.t);
+    if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+  }
+  if (s->setup) (void)hipStreamDestroy(s->setup);
   delete s;
   return M2_OK;
 }
@@ -855,67 +1190,24 @@ int m2_server_submit(m2_server* s, uint64_t p, int mode, uint64_t iters, uint64_
 int m2_server_step(m2_server* s, int* busy) {
   if (!s) return fail(M2_E_INVAL, "null server");
   HIPCHK(hipSetDevice(s->ctx->device));
-  // 1. admit queued requests in arrival order while their footprint fits
+  // Wait up to about a millisecond for something to happen, so that the
+  // caller can submit arrivals in between.
+  const auto t0 = std::chrono::steady_clock::now();
   for (;;) {
-    m2_server::Req rq;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      if (s->queue.empty()) break;
-      rq = s->queue.front();
-    }
-    uint64_t fp = 0;
-    m2_test_footprint(rq.p, &fp);
-    if (s->in_use + fp > s->budget && !s->active.empty()) break;
-    Class* cls = nullptr;
-    int rc = get_class(s->ctx, transform_log2(rq.p), s->ls_cap, &cls);
+    bool progressed = false;
+    int rc = server_admit(s, &progressed);
     if (rc) return rc;
-    auto it = s->groups.find(cls);
-    if (it == s->groups.end()) {
-      Group g;
-      if ((rc = g.create(cls))) return rc;
-      it = s->groups.emplace(cls, g).first;
-    }
-    TestObj* t = nullptr;
-    if ((rc = test_obj_create(s->ctx, rq.p, rq.mode, s->ls_cap, it->second.stream, &t))) return rc;
-    if (rq.iters) t->target = std::min<uint64_t>(t->target, rq.iters);
-    s->active.push_back({rq.id, t, fp});
-    s->in_use += fp;
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->queue.pop_front();
+    for (auto& kv : s->lanes)
+      if ((rc = server_service_lane(s, kv.second, &progressed))) return rc;
+    bool inflight = false;
+    for (auto& kv : s->lanes) inflight |= kv.second.inflight;
+    if (progressed || !inflight) break;
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(1000)) break;
+    std::this_thread::sleep_for(std::chrono::microseconds(20));
   }
-  // 2. one time slice for every active test, each length on its own stream
-  std::map<Class*, std::vector<TestObj*>> by;
-  for (auto& a : s->active) by[a.t->cls].push_back(a.t);
-  for (auto& kv : by) {
-    int rc = s->groups[kv.first].advance(kv.second, s->slice, GRAPH_ITERS);
-    if (rc) return rc;
-  }
-  for (auto& kv : by) {
-    int rc = s->groups[kv.first].sync();
-    if (rc) return rc;
-  }
-  // 3. retire finished tests
-  std::vector<m2_server::Active> keep;
-  for (auto& a : s->active) {
-    if (a.t->iter >= a.t->target) {
-      std::vector<uint8_t> r;
-      int rc = test_obj_residue(a.t, r);
-      if (rc) return rc;
-      {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->done.push_back({a.id, res64_of(r), verdict_of(a.t, r)});
-      }
-      s->in_use -= a.bytes;
-      if ((rc = s->groups[a.t->cls].forget(a.t))) return rc;
-      test_obj_destroy(a.t);
-    } else {
-      keep.push_back(a);
-    }
-  }
-  s->active.swap(keep);
   if (busy) {
     std::lock_guard<std::mutex> lk(s->mu);
-    *busy = (!s->active.empty() || !s->queue.empty()) ? 1 : 0;
+    *busy = (s->nactive > 0 || !s->queue.empty()) ? 1 : 0;
   }
   return M2_OK;
 }
@@ -940,7 +1232,7 @@ int m2_server_stats(m2_server* s, uint64_t* queued, uint64_t* active, uint64_t* 
   if (!s) return fail(M2_E_INVAL, "null server");
   std::lock_guard<std::mutex> lk(s->mu);
   if (queued) *queued = s->queue.size();
-  if (active) *active = s->active.size();
+  if (active) *active = s->nactive;
   if (bytes_in_use) *bytes_in_use = s->in_use;
   return M2_OK;
 }

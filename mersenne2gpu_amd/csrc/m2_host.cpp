@@ -50,9 +50,10 @@ uint32_t bitrev(uint32_t j, int bits) {
 
 int log2i(uint32_t r) { return r == 8 ? 3 : (r == 4 ? 2 : 1); }
 
-// Levels [ell0, ell0 + A): one radix-2 or radix-4 step for A mod 3 levels,
-// then radix-8 steps. Twiddle blocks are appended to tw61/tw31.
-Pass make_pass(int ell0, int A, std::vector<e61>& tw61, std::vector<e31>& tw31) {
+// Levels [ell0, ell0 + A). max_radix 8: one radix-2 or radix-4 step for
+// A mod 3 levels, then radix-8 steps; max_radix 4: one radix-2 step if A is
+// odd, then radix-4 steps. Twiddle blocks are appended to tw61/tw31.
+Pass make_pass(int ell0, int A, std::vector<e61>& tw61, std::vector<e31>& tw31, int max_radix = 8) {
   Pass ps{};
   ps.ell0 = ell0;
   ps.A = A;
@@ -92,6 +93,11 @@ Pass make_pass(int ell0, int A, std::vector<e61>& tw61, std::vector<e31>& tw31) 
     }
     ell += lr;
   };
+  if (max_radix == 4) {
+    if (A % 2) add_step(2);
+    while (ell < ell0 + A) add_step(4);
+    return ps;
+  }
   const int rem = A % 3;
   if (rem == 1) add_step(2);
   if (rem == 2) add_step(4);
@@ -101,11 +107,17 @@ Pass make_pass(int ell0, int A, std::vector<e61>& tw61, std::vector<e31>& tw31) 
 
 }  // namespace
 
-bool build_class_plan(int ln, uint32_t ls_cap, DevPlan& P, ClassTables& T, std::string& err) {
+bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, std::string& err) {
   if (ln < 3 || ln > 27) {
     err = "transform length 2^" + std::to_string(ln) + " outside 2^3..2^27";
     return false;
   }
+  if (opts.max_radix != 0 && opts.max_radix != 4 && opts.max_radix != 8) {
+    err = "max radix must be 4 or 8";
+    return false;
+  }
+  const uint32_t ls_cap = opts.ls_cap;
+  const int maxr = opts.max_radix ? (int)opts.max_radix : 8;
   const uint64_t n = uint64_t(1) << ln, h = n / 2;
   P = DevPlan{};
   P.ln = ln;
@@ -125,7 +137,7 @@ bool build_class_plan(int ln, uint32_t ls_cap, DevPlan& P, ClassTables& T, std::
   }
   if (L <= SMALL_LOG_H && !(ls_cap && L >= c + 2)) {
     P.small = true;
-    P.outer = make_pass(0, L, T.tw61, T.tw31);
+    P.outer = make_pass(0, L, T.tw61, T.tw31, maxr);
     P.nseg = (uint32_t)(h / SEG_ELEMS);
   } else {
     if (L < c + 2) {
@@ -139,20 +151,25 @@ bool build_class_plan(int ln, uint32_t ls_cap, DevPlan& P, ClassTables& T, std::
       a_mid = a / 2;
       a_outer = a - a_mid;
     }
-    P.outer = make_pass(0, a_outer, T.tw61, T.tw31);
+    P.outer = make_pass(0, a_outer, T.tw61, T.tw31, maxr);
     P.logS = L - a_outer;
     P.logW = std::min(11 - a_outer, P.logS);
+    if (opts.carry_pairs) {
+      int lw = 0;
+      while ((2u << lw) <= opts.carry_pairs) ++lw;
+      P.logW = std::max(2, std::min(lw, P.logW));
+    }
     if ((1 << P.logW) < SEG_ELEMS) {
       err = "plan error: outer pass too tall";
       return false;
     }
     if (a_mid) {
       P.has_mid = true;
-      P.mid = make_pass(a_outer, a_mid, T.tw61, T.tw31);
+      P.mid = make_pass(a_outer, a_mid, T.tw61, T.tw31, maxr);
       P.mid_logS = L - a_outer - a_mid;
       P.mid_logW = std::min(11 - a_mid, P.mid_logS);
     }
-    P.centre = make_pass(L - c, c, T.tw61, T.tw31);
+    P.centre = make_pass(L - c, c, T.tw61, T.tw31, maxr);
     P.nseg = (uint32_t)(h >> P.logW);
   }
   // centre twiddles: w_h^bitrev(j) for position 2j (the reference's w[h/2+j])

@@ -87,10 +87,24 @@ struct TestTables {
   std::vector<uint32_t> bitpos;  // n + 1: ceil(p j / n)
 };
 
-// Plan and twiddle tables for transform length 2^ln. ls_cap (the -LS option):
-// 0 = automatic; otherwise the centre block is the largest power of two
-// <= ls_cap (at least 4) and the multi-pass plan is used.
-bool build_class_plan(int ln, uint32_t ls_cap, DevPlan& P, ClassTables& T, std::string& err);
+// Plan choices for one transform length; zero fields mean automatic.
+//   ls_cap      (-LS) the centre block is the largest power of two <= ls_cap
+//               (at least 4) and the multi-pass plan is used;
+//   carry_pairs (-B)  digit pairs per strip row of the outer pass (the carry
+//               block), rounded down to a power of two and clamped to what
+//               the outer pass's row count allows;
+//   max_radix   8 (radix-8 steps) or 4 (radix-4 steps).
+struct PlanOpts {
+  uint32_t ls_cap = 0, carry_pairs = 0, max_radix = 0;
+  bool operator<(const PlanOpts& o) const {
+    if (ls_cap != o.ls_cap) return ls_cap < o.ls_cap;
+    if (carry_pairs != o.carry_pairs) return carry_pairs < o.carry_pairs;
+    return max_radix < o.max_radix;
+  }
+};
+
+// Plan and twiddle tables for transform length 2^ln.
+bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, std::string& err);
 // Digit tables and walk constants of exponent p on plan P (checks the walk
 // against the tables for every digit, and the carry bound of the plan).
 bool build_test_tables(uint32_t p, const DevPlan& P, TestTables& T, DigitGen& D, std::string& err);

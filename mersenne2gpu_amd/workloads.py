@@ -153,3 +153,36 @@ C4_RANGE = (5381, 2424827)        # n = 2^8 .. 2^16
 def batch_exponents(seed: int, count: int = 512, lo: int = C4_RANGE[0], hi: int = C4_RANGE[1]) -> List[int]:
     g = MT19937_64(seed)
     return [draw_log_uniform(g, lo, hi) for _ in range(count)]
+
+
+def log_uniform_over_lengths(g: MT19937_64, ln_lo: int, ln_hi: int) -> int:
+    """Exponent log-uniform over the exponent ranges of n = 2^ln_lo .. 2^ln_hi."""
+    lo = exponent_range(ln_lo)[0]
+    hi = exponent_range(ln_hi)[1]
+    return draw_log_uniform(g, lo, hi)
+
+
+def serve_stream(seed: int, count: int = 2000, rate: float = 40.0, iters: int = 2000,
+                 prp_frac: float = 0.7) -> List[Tuple[float, int, str, int]]:
+    """Poisson arrival stream of LL/PRP requests: (arrival_s, p, mode, iterations).
+
+    Inter-arrival gaps are exponential with mean 1/rate; 70% PRP; exponents
+    drawn so that 80% fall in n = 2^10..2^14, 18% in 2^16..2^20 and 2% at
+    n = 2^22 (log-uniform within each band). Each request runs
+    min(iters, full test) iterations (iters = 0: the full test)."""
+    g = MT19937_64(seed)
+    out = []
+    t = 0.0
+    for _ in range(count):
+        t += -math.log(1.0 - g.uniform()) / rate
+        mode = "prp" if g.uniform() < prp_frac else "ll"
+        u = g.uniform()
+        if u < 0.80:
+            p = log_uniform_over_lengths(g, 10, 14)
+        elif u < 0.98:
+            p = log_uniform_over_lengths(g, 16, 20)
+        else:
+            p = log_uniform_over_lengths(g, 22, 22)
+        full = p if mode == "prp" else p - 2
+        out.append((t, p, mode, min(full, iters) if iters else full))
+    return out

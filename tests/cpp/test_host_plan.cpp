@@ -26,14 +26,19 @@ int main() {
   std::mt19937_64 rng(42);
   const uint32_t ps[] = {3, 5, 7, 127, 349, 353, 607, 1279, 19937, 20983, 21001, 132049, 2424827, 27899687};
   const uint32_t caps[] = {0, 4, 16, 64, 256};
+  const uint32_t variants[][2] = {{0, 8}, {16, 8}, {0, 4}, {4, 4}};  // (carry_pairs, max_radix)
   for (uint32_t p : ps) {
-    for (uint32_t cap : caps) {
+    for (uint32_t cap : caps) for (const auto& var : variants) {
       DevPlan P;
       ClassTables CT;
       TestTables T;
       DigitGen D;
       std::string err;
-      const bool okc = build_class_plan(transform_log2(p), cap, P, CT, err);
+      PlanOpts po;
+      po.ls_cap = cap;
+      po.carry_pairs = var[0];
+      po.max_radix = var[1];
+      const bool okc = build_class_plan(transform_log2(p), po, P, CT, err);
       CHECK(okc, "p=%u cap=%u class plan: %s", p, cap, err.c_str());
       if (!okc) continue;
       CHECK(CT.sq61.size() == P.h / 2 && CT.sq31.size() == P.h / 2, "p=%u centre table size", p);
@@ -58,7 +63,19 @@ int main() {
         CHECK(tgt.count(0) == 1, "p=%u no carry wraps to digit 0", p);
       }
       // residue round trip on random digits, including negatives and carries
-      if (cap == 0 && n <= (1u << 16)) {
+      // every pass's steps stay within the kernels' step table, and cover the pass's levels
+      for (const Pass* ps : {&P.outer, &P.mid, &P.centre}) {
+        if (ps->A == 0) continue;
+        CHECK(ps->nsteps >= 1 && ps->nsteps <= MAX_STEPS, "p=%u steps %d", p, ps->nsteps);
+        int lv = 0;
+        for (int k = 0; k < ps->nsteps; ++k) {
+          CHECK(ps->steps[k].radix <= (int)(var[1]), "p=%u radix %d", p, ps->steps[k].radix);
+          lv += ps->steps[k].radix == 8 ? 3 : (ps->steps[k].radix == 4 ? 2 : 1);
+        }
+        CHECK(lv == ps->A, "p=%u levels %d != %d", p, lv, ps->A);
+      }
+      if (!P.small) CHECK((1 << P.logW) >= SEG_ELEMS && (P.outer.A + P.logW) <= 11, "p=%u outer tile", p);
+      if (cap == 0 && var[0] == 0 && var[1] == 8 && n <= (1u << 16)) {
         std::vector<int64_t> d(n);
         for (uint64_t j = 0; j < n; ++j) d[j] = (int64_t)(rng() % (1ull << T.dig[j].width));
         std::vector<uint8_t> r = residue_from_state(p, T.bitpos, d.data(), n, nullptr, nullptr, 0);

@@ -5,6 +5,7 @@ Python integers), never from the transform code under test. Usage:
     python tests/golden/gen_fixtures.py kat sweep c1 c4      # seconds to a minute
     python tests/golden/gen_fixtures.py c2                   # a few minutes (GMP)
     python tests/golden/gen_fixtures.py c3                   # ~20-30 minutes (GMP)
+    python tests/golden/gen_fixtures.py c5                   # a few minutes (GMP)
 """
 from __future__ import annotations
 
@@ -112,6 +113,27 @@ def gen_c4():
         c["final"] = n
         cases.append(c)
     write("c4.json", {"desc": "512 batched PRP tests, seed 1, log-uniform over n=2^8..2^16",
+                      "seed": 1, "cases": cases})
+
+
+def gen_c5():
+    # Serving stream (seed 1, 2000 requests, 2000-iteration budget): exact Res64
+    # of every request with n <= 2^14 among the first 400, and of the first 6
+    # requests with n = 2^16.
+    reqs = W.serve_stream(1)
+    cases = []
+    seen = set()
+    n16 = 0
+    for i, (_, p, mode, k) in enumerate(reqs):
+        ln = W.transformsize(p)
+        take = (i < 400 and ln <= 14) or (ln == 16 and n16 < 6)
+        if not take or (p, mode, k) in seen:
+            continue
+        n16 += ln == 16
+        seen.add((p, mode, k))
+        cases.append(case(p, mode, [k]))
+    write("c5.json", {"desc": "serving stream seed 1: exact Res64 of requests with n <= 2^14 among the first "
+                              "400, and of the first six with n = 2^16, at their iteration budget",
                       "seed": 1, "cases": cases})
 
 

@@ -158,3 +158,32 @@ def test_server_one_request_larger_than_budget_runs_alone(gpu_ctx):
         out.update({rid: r64 for rid, r64, _ in s.poll()})
         assert out[a] == c["checkpoints"]["50"]["res64"]
         assert out[b] == c["checkpoints"]["200"]["res64"]
+
+
+@pytest.mark.parametrize("variant", [dict(carry_pairs=16), dict(max_radix=4), dict(max_radix=4, carry_pairs=4)])
+def test_plan_variants_match(gpu_ctx, variant):
+    """Carry-block width and radix choices (what the autotuner varies) leave
+    every residue unchanged, on the single-workgroup and multi-pass plans."""
+    for c in _sweep_cases():
+        if not (5 <= c["ln"] <= 17 and c["mode"] == "prp"):
+            continue
+        for ls in (0, 64):
+            if ls and (c["p"] >> c["ln"]) < 12:
+                continue
+            with m2.Test(gpu_ctx, c["p"], "prp", local_stage_cap=ls, **variant) as t:
+                t.run(50)
+                assert t.res64() == c["checkpoints"]["50"]["res64"], (c["p"], ls, variant)
+
+
+def test_autotune_and_tuned_batch(gpu_ctx):
+    """The tuner reports every candidate and picks one; a batch built with
+    autotune=1 gives the exact results."""
+    c16 = [c for c in _sweep_cases() if c["ln"] == 16 and c["mode"] == "prp"][0]
+    opts, report = m2.autotune(gpu_ctx, c16["p"], 4)
+    print(report)
+    assert "best:" in report and opts["max_radix"] in (0, 4, 8)
+    cases = [c for c in _sweep_cases() if 8 <= c["ln"] <= 16]
+    with m2.Batch(gpu_ctx, [c["p"] for c in cases], [c["mode"] for c in cases], autotune=1) as b:
+        b.run(50)
+        for i, c in enumerate(cases):
+            assert b.res64(i) == c["checkpoints"]["50"]["res64"], (c["p"], c["mode"])
