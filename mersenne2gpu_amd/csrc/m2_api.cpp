@@ -136,6 +136,12 @@ int get_class(m2_ctx* ctx, int ln, const PlanOpts& po, Class** out) {
   c->P.sq61 = (const e61*)d;
   if ((rc = dev_alloc(c->allocs, c->bytes, &d, T.sq31.data(), T.sq31.size() * sizeof(e31)))) return bail(rc);
   c->P.sq31 = (const e31*)d;
+  if (!c->P.small) {
+    if ((rc = dev_alloc(c->allocs, c->bytes, &d, T.ctw61.data(), T.ctw61.size() * sizeof(e61)))) return bail(rc);
+    c->P.ctw61 = (const e61*)d;
+    if ((rc = dev_alloc(c->allocs, c->bytes, &d, T.ctw31.data(), T.ctw31.size() * sizeof(e31)))) return bail(rc);
+    c->P.ctw31 = (const e31*)d;
+  }
   *out = c.get();
   ctx->classes[key] = std::move(c);
   return M2_OK;

@@ -178,4 +178,173 @@ template <class E> M2_HD void dft8_inv(E* v) {
   v[3] = add(y3, y7); v[7] = sub(y3, y7);
 }
 
+// ---- twiddle products and lazily reduced butterflies (transform steps) ----
+//
+// Twiddle tables hold M61 values in split form (c1 << 32) | c0 with
+// c = c1 2^31 + c0, c0 < 2^31: then a c + b d for a, b < 2^62 sums its
+// low, middle and high partial products in 64 bits without overflow and
+// needs one fold of the low part plus two final folds.
+M2_HD uint64_t split31(uint64_t c) { return ((c >> 31) << 32) | (c & 0x7FFFFFFFull); }
+M2_HD uint64_t unsplit31(uint64_t w) { return ((w >> 32) << 31) | (w & 0x7FFFFFFFull); }
+M2_HD e61 split31(e61 w) { return {split31(w.a), split31(w.b)}; }
+
+constexpr uint64_t P61x2 = 2 * P61, P61x4 = 4 * P61, P61x8 = 8 * P61;
+
+// a c + b d (mod P61) in [0, P61]: a, b < 2^62; c, d in split form.
+//   a c = a0 c0 + (a0 c1 + 2 a1 c0) 2^31 + a1 c1 2^63, with 2^63 = 4 and
+//   2^61 = 1 (mod P61); the middle sum M contributes (M >> 30) + (M mod 2^30) 2^31.
+M2_HD uint64_t dot61s(uint64_t a, uint64_t c, uint64_t b, uint64_t d) {
+  const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32), b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+  const uint32_t c0 = uint32_t(c), c1 = uint32_t(c >> 32), d0 = uint32_t(d), d1 = uint32_t(d >> 32);
+  const uint64_t lo = uint64_t(a0) * c0 + uint64_t(b0) * d0;  // each < 2^63
+  const uint64_t mid = uint64_t(a0) * c1 + uint64_t(2u * a1) * c0 + uint64_t(b0) * d1 + uint64_t(2u * b1) * d0;
+  const uint64_t hi = uint64_t(a1) * c1 + uint64_t(b1) * d1;  // < 2^61
+  const uint64_t r = fold61(lo) + (hi << 2) + (mid >> 30) + ((mid & 0x3FFFFFFFull) << 31);  // < 2^64
+  return fold61(fold61(r));
+}
+// x w and x conj(w) for x with components <= 2 P61 (P61x2 - b negates b),
+// w a split twiddle; results in [0, P61].
+M2_HD e61 mul_tw(e61 x, e61 w) { return {dot61s(x.a, w.a, P61x2 - x.b, w.b), dot61s(x.a, w.b, x.b, w.a)}; }
+M2_HD e61 mulconj_tw(e61 x, e61 w) { return {dot61s(x.a, w.a, x.b, w.b), dot61s(x.b, w.a, P61x2 - x.a, w.b)}; }
+
+// x mod P31 in [0, P31] for x < 2^63 (2^32 = 2 mod P31).
+M2_HD uint32_t red31(uint64_t x) {
+  const uint64_t t = (x >> 32) * 2u + (x & 0xFFFFFFFFull);  // < 2^33
+  const uint32_t r = uint32_t(t & P31) + uint32_t(t >> 31);
+  return umin32(r, r - P31);
+}
+// GF(M31^2) twiddle products as two 2-term dot products each.
+M2_HD e31 mul_tw(e31 x, e31 w) {
+  const uint32_t nd = P31 - w.b;
+  return {red31(uint64_t(x.a) * w.a + uint64_t(x.b) * nd), red31(uint64_t(x.a) * w.b + uint64_t(x.b) * w.a)};
+}
+M2_HD e31 mulconj_tw(e31 x, e31 w) {
+  const uint32_t nd = P31 - w.b;
+  return {red31(uint64_t(x.a) * w.a + uint64_t(x.b) * w.b), red31(uint64_t(x.b) * w.a + uint64_t(x.a) * nd)};
+}
+
+// x 2^30 (mod P61) for any x < 2^64, result < 2^61 + 2^33.
+M2_HD uint64_t rot30_lazy(uint64_t x) { return (x >> 31) + ((x & 0x7FFFFFFFull) << 30); }
+M2_HD uint64_t red61(uint64_t x) { return fold61(fold61(x)); }  // [0, 2^64) -> [0, P61]
+
+// Forward butterflies over GF(M61^2) with sums kept unreduced between levels
+// (bounds in the comments, P = P61). Inputs: element 0 <= P + 7 (the output
+// of a previous forward step), the others <= P (twiddle products). Outputs
+// are folded once: <= P + 7.
+M2_HD void dft2_fwd_lazy(e61& x0, e61& x1) {
+  const e61 t = x0;
+  x0 = {fold61(t.a + x1.a), fold61(t.b + x1.b)};
+  x1 = {fold61(t.a + (P61 - x1.a)), fold61(t.b + (P61 - x1.b))};
+}
+M2_HD void dft4_fwd_lazy(e61* v) {
+  const uint64_t y0a = v[0].a + v[2].a, y0b = v[0].b + v[2].b;                    // <= 2P + 7
+  const uint64_t y2a = v[0].a + (P61 - v[2].a), y2b = v[0].b + (P61 - v[2].b);
+  const uint64_t y1a = v[1].a + v[3].a, y1b = v[1].b + v[3].b;                    // <= 2P
+  const uint64_t y3a = v[3].b + (P61 - v[1].b), y3b = v[1].a + (P61 - v[3].a);    // i (v1 - v3)
+  v[0] = {fold61(y0a + y1a), fold61(y0b + y1b)};
+  v[1] = {fold61(y0a + (P61x2 - y1a)), fold61(y0b + (P61x2 - y1b))};
+  v[2] = {fold61(y2a + y3a), fold61(y2b + y3b)};
+  v[3] = {fold61(y2a + (P61x2 - y3a)), fold61(y2b + (P61x2 - y3b))};
+}
+M2_HD void dft8_fwd_lazy(e61* v) {
+  // level 1
+  const uint64_t y0a = v[0].a + v[4].a, y0b = v[0].b + v[4].b;                    // <= 2P + 7
+  const uint64_t y4a = v[0].a + (P61 - v[4].a), y4b = v[0].b + (P61 - v[4].b);
+  const uint64_t y1a = v[1].a + v[5].a, y1b = v[1].b + v[5].b;                    // <= 2P
+  const uint64_t y5a = v[1].a + (P61 - v[5].a), y5b = v[1].b + (P61 - v[5].b);
+  const uint64_t y2a = v[2].a + v[6].a, y2b = v[2].b + v[6].b;
+  const uint64_t y6a = v[6].b + (P61 - v[2].b), y6b = v[2].a + (P61 - v[6].a);    // i (v2 - v6)
+  const uint64_t y3a = v[3].a + v[7].a, y3b = v[3].b + v[7].b;
+  const uint64_t y7a = v[7].b + (P61 - v[3].b), y7b = v[3].a + (P61 - v[7].a);    // i (v3 - v7)
+  // level 2
+  const uint64_t z0a = y0a + y2a, z0b = y0b + y2b;                                // <= 4P + 7
+  const uint64_t z2a = y0a + (P61x2 - y2a), z2b = y0b + (P61x2 - y2b);
+  const uint64_t z1a = y1a + y3a, z1b = y1b + y3b;                                // <= 4P
+  const uint64_t z3a = y3b + (P61x2 - y1b), z3b = y1a + (P61x2 - y3a);            // i (y1 - y3)
+  const uint64_t z4a = y4a + y6a, z4b = y4b + y6b;
+  const uint64_t z6a = y4a + (P61x2 - y6a), z6b = y4b + (P61x2 - y6b);
+  const uint64_t sa = y5a + y7a, sb = y5b + y7b;                                  // <= 4P
+  const uint64_t z5a = rot30_lazy(sb + (P61x4 - sa)), z5b = rot30_lazy(P61x8 - sa - sb);  // w8 s
+  const uint64_t ta = y7b + (P61x2 - y5b), tb = y5a + (P61x2 - y7a);              // i (y5 - y7)
+  const uint64_t z7a = rot30_lazy(tb + (P61x4 - ta)), z7b = rot30_lazy(P61x8 - ta - tb);  // w8 t
+  // level 3 (sums < 2^64)
+  v[0] = {fold61(z0a + z1a), fold61(z0b + z1b)};
+  v[1] = {fold61(z0a + (P61x4 - z1a)), fold61(z0b + (P61x4 - z1b))};
+  v[2] = {fold61(z2a + z3a), fold61(z2b + z3b)};
+  v[3] = {fold61(z2a + (P61x4 - z3a)), fold61(z2b + (P61x4 - z3b))};
+  v[4] = {fold61(z4a + z5a), fold61(z4b + z5b)};
+  v[5] = {fold61(z4a + (P61x2 - z5a)), fold61(z4b + (P61x2 - z5b))};
+  v[6] = {fold61(z6a + z7a), fold61(z6b + z7b)};
+  v[7] = {fold61(z6a + (P61x2 - z7a)), fold61(z6b + (P61x2 - z7b))};
+}
+// Inverse butterflies: inputs <= P; output 0 fully reduced (<= P), the others
+// folded once (< 2^62, ready for mulconj_tw).
+M2_HD void dft2_inv_lazy(e61& x0, e61& x1) {
+  const e61 t = x0;
+  x0 = {fold61(t.a + x1.a), fold61(t.b + x1.b)};
+  x1 = {fold61(t.a + (P61 - x1.a)), fold61(t.b + (P61 - x1.b))};
+}
+M2_HD void dft4_inv_lazy(e61* v) {
+  const uint64_t z0a = v[0].a + v[1].a, z0b = v[0].b + v[1].b;                    // <= 2P
+  const uint64_t z1a = v[0].a + (P61 - v[1].a), z1b = v[0].b + (P61 - v[1].b);
+  const uint64_t z2a = v[2].a + v[3].a, z2b = v[2].b + v[3].b;
+  const uint64_t z3a = v[2].b + (P61 - v[3].b), z3b = v[3].a + (P61 - v[2].a);    // -i (v2 - v3)
+  v[0] = {red61(z0a + z2a), red61(z0b + z2b)};
+  v[2] = {fold61(z0a + (P61x2 - z2a)), fold61(z0b + (P61x2 - z2b))};
+  v[1] = {fold61(z1a + z3a), fold61(z1b + z3b)};
+  v[3] = {fold61(z1a + (P61x2 - z3a)), fold61(z1b + (P61x2 - z3b))};
+}
+M2_HD void dft8_inv_lazy(e61* v) {
+  // level 1
+  const uint64_t z0a = v[0].a + v[1].a, z0b = v[0].b + v[1].b;                    // <= 2P
+  const uint64_t z1a = v[0].a + (P61 - v[1].a), z1b = v[0].b + (P61 - v[1].b);
+  const uint64_t z2a = v[2].a + v[3].a, z2b = v[2].b + v[3].b;
+  const uint64_t z3a = v[2].b + (P61 - v[3].b), z3b = v[3].a + (P61 - v[2].a);    // -i (v2 - v3)
+  const uint64_t z4a = v[4].a + v[5].a, z4b = v[4].b + v[5].b;
+  const uint64_t da = v[4].a + (P61 - v[5].a), db = v[4].b + (P61 - v[5].b);
+  const uint64_t z5a = rot30_lazy(P61x4 - da - db), z5b = rot30_lazy(da + (P61x2 - db));  // conj(w8) d
+  const uint64_t z6a = v[6].a + v[7].a, z6b = v[6].b + v[7].b;
+  const uint64_t ma = v[6].b + (P61 - v[7].b), mb = v[7].a + (P61 - v[6].a);      // -i (v6 - v7)
+  const uint64_t z7a = rot30_lazy(P61x4 - ma - mb), z7b = rot30_lazy(ma + (P61x2 - mb));
+  // level 2 (z5, z7 < 2^61 + 2^33)
+  const uint64_t y0a = z0a + z2a, y0b = z0b + z2b;                                // <= 4P
+  const uint64_t y2a = z0a + (P61x2 - z2a), y2b = z0b + (P61x2 - z2b);
+  const uint64_t y1a = z1a + z3a, y1b = z1b + z3b;
+  const uint64_t y3a = z1a + (P61x2 - z3a), y3b = z1b + (P61x2 - z3b);
+  const uint64_t y4a = z4a + z6a, y4b = z4b + z6b;
+  const uint64_t y6a = z4b + (P61x2 - z6b), y6b = z6a + (P61x2 - z4a);            // -i (z4 - z6)
+  const uint64_t y5a = z5a + z7a, y5b = z5b + z7b;                                // < 2P
+  const uint64_t y7a = z5b + (P61x2 - z7b), y7b = z7a + (P61x2 - z5a);            // -i (z5 - z7)
+  // level 3 (sums <= 8P)
+  v[0] = {red61(y0a + y4a), red61(y0b + y4b)};
+  v[4] = {fold61(y0a + (P61x4 - y4a)), fold61(y0b + (P61x4 - y4b))};
+  v[1] = {fold61(y1a + y5a), fold61(y1b + y5b)};
+  v[5] = {fold61(y1a + (P61x4 - y5a)), fold61(y1b + (P61x4 - y5b))};
+  v[2] = {fold61(y2a + y6a), fold61(y2b + y6b)};
+  v[6] = {fold61(y2a + (P61x4 - y6a)), fold61(y2b + (P61x4 - y6b))};
+  v[3] = {fold61(y3a + y7a), fold61(y3b + y7b)};
+  v[7] = {fold61(y3a + (P61x4 - y7a)), fold61(y3b + (P61x4 - y7b))};
+}
+
+// Real-data pairing and squaring of one pair of spectrum values (the
+// reference's square_half, mersenne2_gpu.cpp:856-870): with
+// u0 = z_k + conj(z_mk), u1 = z_k - conj(z_mk), v0 = u0^2 - u1^2 tw,
+// v1 = 2 u0 u1, it sets z_k <- v0 + v1, z_mk <- conj(v0 - v1). Since
+// u0 + u1 = 2 z_k and u0 - u1 = 2 conj(z_mk), the same values are
+//   z_k <- 4 z_k^2 - C,  z_mk <- 4 z_mk^2 - conj(C),  C = u1^2 (1 + tw),
+// three squarings and one twiddle product; tw1 = 1 + tw (split form for M61).
+M2_HD e61 red(e61 x) { return {fold61(x.a), fold61(x.b)}; }  // <= P61 + 7 -> <= P61
+M2_HD e31 red(e31 x) { return x; }
+M2_HD e61 times4(e61 x) { return {rot61(x.a, 2), rot61(x.b, 2)}; }
+M2_HD e31 times4(e31 x) { return {rot31(x.a, 2), rot31(x.b, 2)}; }
+template <class E>
+M2_HD void square_pair(E& zk, E& zmk, E tw1) {
+  const E a = red(zk), b = red(zmk);
+  const E u1 = sub(a, conj(b));
+  const E C = mul_tw(sqr(u1), tw1);
+  zk = sub(times4(sqr(a)), C);
+  zmk = sub(times4(sqr(b)), conj(C));
+}
+
+
 }  // namespace m2

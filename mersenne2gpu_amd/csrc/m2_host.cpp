@@ -189,10 +189,49 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
       c61 = mul(c61, w61);
       c31 = mul(c31, w31);
     }
+    // stored as 1 + w (the centre's squaring uses u1^2 (1 + w)), M61 in split form
+    const e61 one61{1, 0};
+    const e31 one31{1, 0};
     for (uint64_t j = 0; j < h / 2; ++j) {
       const uint32_t br = bits > 0 ? bitrev((uint32_t)j, bits) : 0u;
-      T.sq61[j] = p61[br];
-      T.sq31[j] = p31[br];
+      T.sq61[j] = split31(add(one61, p61[br]));
+      T.sq31[j] = add(one31, p31[br]);
+    }
+  }
+  // step twiddles: M61 in the split form of mul_tw
+  for (e61& w : T.tw61) w = split31(w);
+  // per-workgroup centre records (contiguous, so a workgroup stages its
+  // twiddles with one batched copy)
+  if (!P.small) {
+    const Pass& cp = P.centre;
+    const int c = cp.A;
+    const uint32_t ncentre = (uint32_t)((h >> c) / 2);
+    uint32_t nT = 0;
+    for (int si = 0; si < cp.nsteps; ++si)
+      nT += 2u * (1u << (cp.steps[si].ell - cp.ell0)) * (uint32_t)(cp.steps[si].radix - 1);
+    P.crec = nT + (1u << c);
+    T.ctw61.clear();
+    T.ctw31.clear();
+    T.ctw61.reserve((size_t)ncentre * P.crec);
+    T.ctw31.reserve((size_t)ncentre * P.crec);
+    for (uint32_t g = 0; g < ncentre; ++g) {
+      uint32_t bl[2];
+      centre_blocks(g, bl[0], bl[1]);
+      for (int si = 0; si < cp.nsteps; ++si) {
+        const Step& st = cp.steps[si];
+        const int lam = st.ell - cp.ell0;
+        const uint32_t span = (1u << lam) * (uint32_t)(st.radix - 1);
+        for (int b = 0; b < 2; ++b) {
+          const uint32_t g0 = st.twoff + (bl[b] << lam) * (uint32_t)(st.radix - 1);
+          T.ctw61.insert(T.ctw61.end(), T.tw61.begin() + g0, T.tw61.begin() + g0 + span);
+          T.ctw31.insert(T.ctw31.end(), T.tw31.begin() + g0, T.tw31.begin() + g0 + span);
+        }
+      }
+      for (int b = 0; b < 2; ++b) {
+        const size_t s0 = (size_t)bl[b] << (c - 1);
+        T.ctw61.insert(T.ctw61.end(), T.sq61.begin() + s0, T.sq61.begin() + s0 + (1u << (c - 1)));
+        T.ctw31.insert(T.ctw31.end(), T.sq31.begin() + s0, T.sq31.begin() + s0 + (1u << (c - 1)));
+      }
     }
   }
   return true;

@@ -41,7 +41,28 @@ struct DevPlan {
   int mid_logS = 0, mid_logW = 0;
   Pass centre{};
   uint32_t nseg = 0;  // number of carry segments (length of G)
+  // Centre twiddles regrouped per centre workgroup: record g (crec entries)
+  // holds, for its blocks (B, B2), the step twiddles in the order the
+  // workgroup keeps them in LDS, then the squaring twiddles of B and B2.
+  const e61* ctw61 = nullptr;
+  const e31* ctw31 = nullptr;
+  uint32_t crec = 0;
 };
+
+// Blocks (B, B2) squared together by centre workgroup g: (0, 1) for g = 0,
+// else the pair B + B2 = 3 2^a - 1 within octave a (the real-data pairing).
+inline void centre_blocks(uint32_t g, uint32_t& B, uint32_t& B2) {
+  if (g == 0) {
+    B = 0;
+    B2 = 1;
+    return;
+  }
+  uint32_t a = 0;
+  while ((2u << a) <= g) ++a;  // 2^a <= g < 2^(a+1)
+  ++a;                         // blocks [2^a, 2^(a+1))
+  B = (1u << a) + (g - (1u << (a - 1)));
+  B2 = 3u * (1u << a) - 1u - B;
+}
 
 // Position of element e's digit pair between iterations: natural order on the
 // small plan, strip-major (strip cg = column / W, then row, then column
@@ -81,6 +102,8 @@ struct ClassTables {
   std::vector<e31> tw31;
   std::vector<e61> sq61;  // h/2
   std::vector<e31> sq31;
+  std::vector<e61> ctw61;  // multi-pass plan: per-workgroup centre records
+  std::vector<e31> ctw31;
 };
 struct TestTables {
   std::vector<DigitInfo> dig;    // n

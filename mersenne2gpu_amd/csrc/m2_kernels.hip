@@ -155,8 +155,21 @@ __device__ __forceinline__ void seg_weight(e61* L61, e31* L31, int idx0, uint32_
 // LDS at offset lo + b * (1 << lam) * (r-1), lo = sum of the earlier steps.
 __device__ __forceinline__ int twiddle_span(const Step& st, int lam) { return (1 << lam) * (st.radix - 1); }
 
+template <int B, class Src, class Dst>
+__device__ __forceinline__ void gather2(e61* L61, e31* L31, const e61* __restrict__ g61, const e31* __restrict__ g31,
+                                        int count, Src src, Dst dst);
+
 __device__ __forceinline__ int stage_pass_twiddles(e61* T61, e31* T31, const Pass& ps, int nb, const uint32_t* j0,
                                                    const e61* __restrict__ tw61, const e31* __restrict__ tw31) {
+  if (ps.ell0 == 0 && nb == 1 && j0[0] == 0) {
+    // a pass that starts at level 0 uses its steps' whole tables, which lie
+    // back to back: one flat batched copy of 2^A - 1 entries
+    const int n = (1 << ps.A) - 1;
+    const e61* g61 = tw61 + ps.steps[0].twoff;
+    const e31* g31 = tw31 + ps.steps[0].twoff;
+    gather2<4>(T61, T31, g61, g31, n, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+    return n;
+  }
   int lo = 0;
   for (int si = 0; si < ps.nsteps; ++si) {
     const Step st = ps.steps[si];
@@ -174,7 +187,9 @@ __device__ __forceinline__ int stage_pass_twiddles(e61* T61, e31* T31, const Pas
   return lo;
 }
 
-// One radix step over data in LDS. The LDS holds nb blocks of R = 2^logR rows
+// One radix step over data in LDS. M61 values leave forward steps folded
+// once (<= P61 + 7) and inverse steps reduced (<= P61); twiddles are in the
+// split form of mul_tw. The LDS holds nb blocks of R = 2^logR rows
 // by W = 2^logW columns (row-major); every column of a block is an
 // independent transform. Step `lam` (relative level) with radix r: the
 // butterflies combine rows blk*(R>>lam) + off + e*d, e < r, d = R >> (lam +
@@ -209,20 +224,20 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
     if (FWD) {
 #pragma unroll
       for (int e = 1; e < RADIX; ++e) {
-        v[e] = mul(v[e], T61[e - 1]);
-        x[e] = mul(x[e], T31[e - 1]);
+        v[e] = mul_tw(v[e], T61[e - 1]);
+        x[e] = mul_tw(x[e], T31[e - 1]);
       }
-      if (RADIX == 8) { dft8_fwd(v); dft8_fwd(x); }
-      if (RADIX == 4) { dft4_fwd(v); dft4_fwd(x); }
-      if (RADIX == 2) { dft2(v[0], v[1]); dft2(x[0], x[1]); }
+      if (RADIX == 8) { dft8_fwd_lazy(v); dft8_fwd(x); }
+      if (RADIX == 4) { dft4_fwd_lazy(v); dft4_fwd(x); }
+      if (RADIX == 2) { dft2_fwd_lazy(v[0], v[1]); dft2(x[0], x[1]); }
     } else {
-      if (RADIX == 8) { dft8_inv(v); dft8_inv(x); }
-      if (RADIX == 4) { dft4_inv(v); dft4_inv(x); }
-      if (RADIX == 2) { dft2(v[0], v[1]); dft2(x[0], x[1]); }
+      if (RADIX == 8) { dft8_inv_lazy(v); dft8_inv(x); }
+      if (RADIX == 4) { dft4_inv_lazy(v); dft4_inv(x); }
+      if (RADIX == 2) { dft2_inv_lazy(v[0], v[1]); dft2(x[0], x[1]); }
 #pragma unroll
       for (int e = 1; e < RADIX; ++e) {
-        v[e] = mulconj(v[e], T61[e - 1]);
-        x[e] = mulconj(x[e], T31[e - 1]);
+        v[e] = mulconj_tw(v[e], T61[e - 1]);
+        x[e] = mulconj_tw(x[e], T31[e - 1]);
       }
     }
 #pragma unroll
@@ -255,20 +270,6 @@ __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, c
   }
 }
 
-// Real-data pairing and squaring of one pair of spectrum values (the
-// reference's square_half, mersenne2_gpu.cpp:856-870): with
-// u0 = z_k + conj(z_mk), u1 = z_k - conj(z_mk), v0 = u0^2 - u1^2 tw,
-// v1 = 2 u0 u1:  z_k <- v0 + v1,  z_mk <- conj(v0 - v1).
-template <class E>
-__device__ __forceinline__ void square_pair(E& zk, E& zmk, E tw) {
-  const E c = conj(zmk);
-  const E u0 = add(zk, c), u1 = sub(zk, c);
-  const E v0 = sub(sqr(u0), mul(sqr(u1), tw));
-  const E v1 = mul(u0, add(u1, u1));
-  zk = add(v0, v1);
-  zmk = conj(sub(v0, v1));
-}
-
 // Squaring of blocks that pair with themselves: block 0 (positions 0..Cb-1,
 // mirror formula, k = 0 special) at LDS offset 0, and, when nself = 2, block 1
 // (positions Cb..2Cb-1, k pairs with 3Cb-1-k) at LDS offset Cb. S61/S31 hold
@@ -290,7 +291,7 @@ __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int n
       e31 b = L31[PADX(0)], b2 = b;
       square_pair(a, a2, t61);
       square_pair(b, b2, t31);
-      const e61 z1 = L61[PADX(1)];
+      const e61 z1 = red(L61[PADX(1)]);
       const e31 y1 = L31[PADX(1)];
       L61[PADX(0)] = a;
       L31[PADX(0)] = b;
@@ -336,6 +337,43 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, cons
     L31[PADX(mkidx)] = d;
   }
   __syncthreads();
+}
+
+// Copies count elements of both fields, global index src(i) -> LDS index
+// dst(i), with each thread issuing B loads before its LDS stores so that the
+// loads' latencies overlap instead of adding up.
+template <int B, class Src, class Dst>
+__device__ __forceinline__ void gather2(e61* L61, e31* L31, const e61* __restrict__ g61, const e31* __restrict__ g31,
+                                        int count, Src src, Dst dst) {
+  // Full batches have no per-lane conditions, so the staged values stay in
+  // registers; the remainder (count not a multiple of B * blockDim) goes one
+  // element at a time.
+  const int step = B * (int)blockDim.x;
+  int base = 0;
+  for (; base + step <= count; base += step) {
+    uint64_t a[B], b[B];
+    uint32_t c[B], d[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const uint32_t j = src(base + k * (int)blockDim.x + (int)threadIdx.x);
+      a[k] = g61[j].a;
+      b[k] = g61[j].b;
+      c[k] = g31[j].a;
+      d[k] = g31[j].b;
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int j = dst(base + k * (int)blockDim.x + (int)threadIdx.x);
+      L61[j] = e61{a[k], b[k]};
+      L31[j] = e31{c[k], d[k]};
+    }
+  }
+  for (int i = base + (int)threadIdx.x; i < count; i += blockDim.x) {
+    const uint32_t j = src(i);
+    const int t = dst(i);
+    L61[t] = g61[j];
+    L31[t] = g31[j];
+  }
 }
 
 __device__ __forceinline__ void stage_range(e61* S61, e31* S31, const e61* __restrict__ g61,
@@ -409,6 +447,7 @@ __global__ void __launch_bounds__(256) k_outer_fwd(const TestDesc* __restrict__ 
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
+  __shared__ Carry128 gcin[PASS_MAX_ELEMS / SEG_ELEMS];
   const TestDesc td = tests[blockIdx.y];
   const e61* __restrict__ zd = td.zd;
   e61* __restrict__ z61 = td.z61;
@@ -419,9 +458,27 @@ __global__ void __launch_bounds__(256) k_outer_fwd(const TestDesc* __restrict__ 
   const uint32_t cg = blockIdx.x, NS = (1u << logS) >> logW;
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
-  // digits of the strip are contiguous
+  // digits of the strip are contiguous; with them, the carry entering each
+  // strip row (loads batched: up to 8 digit pairs and 2 carries per thread
+  // in flight)
   const e61* src = zd + (size_t)cg * (R * W);
-  for (int i = threadIdx.x; i < R * W; i += blockDim.x) L61[PADX(i)] = src[i];
+  {
+    const int n = R * W, step = 8 * (int)blockDim.x;
+    int base = 0;
+    for (; base + step <= n; base += step) {
+      uint64_t a[8], b[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
+        a[k] = src[i].a;
+        b[k] = src[i].b;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) L61[PADX(base + k * (int)blockDim.x + (int)threadIdx.x)] = e61{a[k], b[k]};
+    }
+    for (int i = base + (int)threadIdx.x; i < n; i += blockDim.x) L61[PADX(i)] = src[i];
+    for (int rr = threadIdx.x; rr < R; rr += blockDim.x) gcin[rr] = G[prev_strip_carry(cg, rr, R, NS)];
+  }
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   // prologue: carry-in of each strip-row, then weight
@@ -430,7 +487,7 @@ __global__ void __launch_bounds__(256) k_outer_fwd(const TestDesc* __restrict__ 
     const int r = it / subs, v = it - r * subs;
     const int idx0 = r * W + v * SEG_ELEMS;
     const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
-    if (v == 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(G, prev_strip_carry(cg, (uint32_t)r, R, NS)));
+    if (v == 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gcin, (uint32_t)r));
     seg_weight(L61, L31, idx0, 2u * e0, D);
   }
   __syncthreads();
@@ -460,11 +517,9 @@ __global__ void __launch_bounds__(256) k_outer_inv(const TestDesc* __restrict__ 
   const uint32_t cg = blockIdx.x;
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
-  for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
-    const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
-    L61[PADX(i)] = z61[e];
-    L31[PADX(i)] = z31[e];
-  }
+  gather2<8>(L61, L31, z61, z31, R * W,
+             [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); },
+             [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   lds_pass<false>(L61, L31, logW, 1, ps, T61, T31);
@@ -525,9 +580,8 @@ __global__ void __launch_bounds__(256) k_mid(const TestDesc* __restrict__ tests,
 // C = 2^logC elements that the real-data pairing links; workgroup 0 takes
 // blocks 0 and 1, which pair with themselves. Forward levels [ps.ell0, L),
 // square, inverse. There are h / (2C) workgroups.
-__global__ void __launch_bounds__(256) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
-                                                const e31* __restrict__ tw31, const e61* __restrict__ sq61,
-                                                const e31* __restrict__ sq31, Pass ps) {
+__global__ void __launch_bounds__(256) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
+                                                const e31* __restrict__ ctw31, uint32_t crec, Pass ps) {
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
   __shared__ e61 T61[2 << CENTRE_LOG_C];
@@ -546,16 +600,16 @@ __global__ void __launch_bounds__(256) k_centre(const TestDesc* __restrict__ tes
     B = (1u << a) + (g - (1u << (a - 1)));
     B2 = 3u * (1u << a) - 1u - B;
   }
-  const uint32_t j0[2] = {B, B2};
-  for (int i = threadIdx.x; i < (nb << logC); i += blockDim.x) {
-    const uint32_t e = ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1));
-    L61[PADX(i)] = z61[e];
-    L31[PADX(i)] = z31[e];
-  }
-  stage_pass_twiddles(T61, T31, ps, nb, j0, tw61, tw31);
-  // centre twiddles of the even positions of each block
-  stage_range(S61, S31, sq61, sq31, B << (logC - 1), C >> 1);
-  stage_range(S61 + (C >> 1), S31 + (C >> 1), sq61, sq31, B2 << (logC - 1), C >> 1);
+  // the blocks' elements, then this workgroup's twiddle record (step
+  // twiddles, then squaring twiddles), all loads batched
+  gather2<8>(L61, L31, z61, z31, nb << logC,
+             [&](int i) { return ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1)); },
+             [](int i) { return PADX(i); });
+  const int nT = (int)crec - C;
+  const e61* r61 = ctw61 + (size_t)g * crec;
+  const e31* r31 = ctw31 + (size_t)g * crec;
+  gather2<8>(T61, T31, r61, r31, nT, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+  gather2<4>(S61, S31, r61 + nT, r31 + nT, C, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
   __syncthreads();
   lds_pass<true>(L61, L31, 0, nb, ps, T61, T31);
   if (g == 0) square_self(L61, L31, logC, 2, S61, S31);
@@ -591,8 +645,7 @@ hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& l
                                P.mid_logW)) != hipSuccess)
     return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
-  if ((e = launch("k_centre", k_centre, dim3(ncentre, nt), tests, P.tw61, P.tw31, P.sq61, P.sq31, P.centre)) !=
-      hipSuccess)
+  if ((e = launch("k_centre", k_centre, dim3(ncentre, nt), tests, P.ctw61, P.ctw31, P.crec, P.centre)) != hipSuccess)
     return e;
   if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid, nt), tests, P.tw61, P.tw31, P.mid, P.mid_logS,
                                P.mid_logW)) != hipSuccess)

@@ -1,0 +1,148 @@
+// CPU checks of the transform arithmetic in m2_arith.h (the same inline
+// functions the gfx950 kernels compile): split-twiddle products, the 64-bit
+// M31 reduction, the lazily reduced M61 butterflies and the centre squaring
+// formula, against plain modular arithmetic on random and extreme inputs.
+#include <stdio.h>
+
+#include <random>
+
+#include "m2_arith.h"
+
+using namespace m2;
+typedef unsigned __int128 u128;
+
+static int fails = 0;
+#define CHECK(c, ...)          \
+  do {                         \
+    if (!(c)) {                \
+      if (fails < 20) {        \
+        printf("FAIL %s: ", #c); \
+        printf(__VA_ARGS__);   \
+        printf("\n");          \
+      }                        \
+      ++fails;                 \
+    }                          \
+  } while (0)
+
+static uint64_t m61(u128 x) { return (uint64_t)(x % P61); }
+static uint64_t canon61(uint64_t x) { return x % P61; }  // P61 (and above) as residues
+static uint32_t canon31(uint32_t x) { return x % P31; }
+static bool eq61(e61 x, e61 y) { return canon61(x.a) == canon61(y.a) && canon61(x.b) == canon61(y.b); }
+static bool eq31(e31 x, e31 y) { return canon31(x.a) == canon31(y.a) && canon31(x.b) == canon31(y.b); }
+
+static std::mt19937_64 rng(12345);
+// values in [0, bound] with the ends over-represented
+static uint64_t pick(uint64_t bound) {
+  switch (rng() % 8) {
+    case 0: return 0;
+    case 1: return bound;
+    case 2: return bound - (rng() % 8 < bound ? rng() % 8 : 0);
+    case 3: return P61 < bound ? P61 : bound;
+    default: return bound == ~0ull ? rng() : rng() % (bound + 1);
+  }
+}
+static uint32_t pick31() {
+  switch (rng() % 6) {
+    case 0: return 0;
+    case 1: return P31;
+    case 2: return P31 - 1;
+    default: return (uint32_t)(rng() % (uint64_t(P31) + 1));
+  }
+}
+static e61 rnd61(uint64_t bound) { return {pick(bound), pick(bound)}; }
+static e31 rnd31() { return {pick31(), pick31()}; }
+static e61 reduced(e61 x) { return {canon61(x.a), canon61(x.b)}; }
+
+int main() {
+  const int N = 200000;
+  // dot61s: a, b < 2^62, c, d <= P61 (split)
+  for (int i = 0; i < N; ++i) {
+    const uint64_t a = pick((1ull << 62) - 1), b = pick((1ull << 62) - 1), c = pick(P61), d = pick(P61);
+    const uint64_t r = dot61s(a, split31(c), b, split31(d));
+    CHECK(r <= P61, "dot61s range %llx", (unsigned long long)r);
+    CHECK(canon61(r) == m61((u128)a * c + (u128)b * d), "dot61s a=%llx b=%llx c=%llx d=%llx",
+          (unsigned long long)a, (unsigned long long)b, (unsigned long long)c, (unsigned long long)d);
+    CHECK(unsplit31(split31(c)) == c, "split round trip");
+  }
+  // twiddle products against the plain GF(p^2) products
+  for (int i = 0; i < N; ++i) {
+    const e61 x = rnd61(P61x2), w = rnd61(P61);
+    const e61 xr = reduced(x);
+    const e61 p = mul_tw(x, split31(w)), q = mulconj_tw(x, split31(w));
+    CHECK(p.a <= P61 && p.b <= P61 && q.a <= P61 && q.b <= P61, "mul_tw range");
+    CHECK(eq61(p, mul(xr, w)), "mul_tw 61");
+    CHECK(eq61(q, mulconj(xr, w)), "mulconj_tw 61");
+    const e31 y = rnd31(), v = rnd31();
+    const e31 s = mul_tw(y, v), t = mulconj_tw(y, v);
+    CHECK(s.a <= P31 && s.b <= P31 && t.a <= P31 && t.b <= P31, "mul_tw 31 range");
+    CHECK(eq31(s, mul(y, v)), "mul_tw 31");
+    CHECK(eq31(t, mulconj(y, v)), "mulconj_tw 31");
+  }
+  for (int i = 0; i < N; ++i) {
+    const uint64_t x = pick((1ull << 63) - 1);
+    CHECK(red31(x) <= P31 && canon31(red31(x)) == x % P31, "red31 %llx", (unsigned long long)x);
+    const uint64_t y = pick(~0ull);
+    CHECK(rot30_lazy(y) < (1ull << 61) + (1ull << 33) && canon61(rot30_lazy(y)) == m61((u128)y << 30),
+          "rot30_lazy");
+  }
+  // lazy butterflies against the reduced generic ones: forward inputs have
+  // element 0 <= P61 + 7, the rest <= P61; inverse inputs <= P61
+  for (int i = 0; i < N / 4; ++i) {
+    e61 v[8], ref[8];
+    for (int e = 0; e < 8; ++e) v[e] = rnd61(e == 0 ? P61 + 7 : P61);
+    for (int e = 0; e < 8; ++e) ref[e] = reduced(v[e]);
+    e61 v4[4], r4[4], v2[2], r2[2];
+    for (int e = 0; e < 4; ++e) { v4[e] = v[e]; r4[e] = ref[e]; }
+    for (int e = 0; e < 2; ++e) { v2[e] = v[e]; r2[e] = ref[e]; }
+    dft8_fwd_lazy(v);
+    dft8_fwd(ref);
+    dft4_fwd_lazy(v4);
+    dft4_fwd(r4);
+    dft2_fwd_lazy(v2[0], v2[1]);
+    dft2(r2[0], r2[1]);
+    for (int e = 0; e < 8; ++e) {
+      CHECK(v[e].a <= P61 + 7 && v[e].b <= P61 + 7, "fwd8 range");
+      CHECK(eq61(v[e], ref[e]), "dft8_fwd_lazy e=%d", e);
+    }
+    for (int e = 0; e < 4; ++e) CHECK(v4[e].a <= P61 + 7 && eq61(v4[e], r4[e]), "dft4_fwd_lazy e=%d", e);
+    for (int e = 0; e < 2; ++e) CHECK(v2[e].a <= P61 + 7 && eq61(v2[e], r2[e]), "dft2_fwd_lazy e=%d", e);
+
+    for (int e = 0; e < 8; ++e) v[e] = ref[e] = rnd61(P61);
+    for (int e = 0; e < 4; ++e) v4[e] = r4[e] = v[e];
+    for (int e = 0; e < 2; ++e) v2[e] = r2[e] = v[e];
+    dft8_inv_lazy(v);
+    dft8_inv(ref);
+    dft4_inv_lazy(v4);
+    dft4_inv(r4);
+    dft2_inv_lazy(v2[0], v2[1]);
+    dft2(r2[0], r2[1]);
+    CHECK(v[0].a <= P61 && v[0].b <= P61 && v4[0].a <= P61 && v4[0].b <= P61, "inv output 0 reduced");
+    CHECK(v2[0].a <= P61 && v2[1].a <= P61 && v2[0].b <= P61 && v2[1].b <= P61, "inv2 reduced");
+    for (int e = 0; e < 8; ++e) {
+      CHECK(v[e].a < (1ull << 62) && v[e].b < (1ull << 62), "inv8 range");
+      CHECK(eq61(v[e], ref[e]), "dft8_inv_lazy e=%d", e);
+    }
+    for (int e = 0; e < 4; ++e) CHECK(eq61(v4[e], r4[e]), "dft4_inv_lazy e=%d", e);
+    for (int e = 0; e < 2; ++e) CHECK(eq61(v2[e], r2[e]), "dft2_inv_lazy e=%d", e);
+  }
+  // centre squaring: 4 z_k^2 - C, 4 z_mk^2 - conj(C) equals the defining
+  // formula v0 +/- v1 with u0 = z_k + conj(z_mk), u1 = z_k - conj(z_mk)
+  for (int i = 0; i < N / 4; ++i) {
+    const e61 zk = rnd61(P61 + 7), zmk = rnd61(P61 + 7), tw = rnd61(P61);
+    e61 a = zk, b = zmk;
+    square_pair(a, b, split31(add(e61{1, 0}, tw)));
+    const e61 k = reduced(zk), c = conj(reduced(zmk));
+    const e61 u0 = add(k, c), u1 = sub(k, c);
+    const e61 v0 = sub(sqr(u0), mul(sqr(u1), tw)), v1 = mul(u0, add(u1, u1));
+    CHECK(a.a <= P61 && a.b <= P61 && b.a <= P61 && b.b <= P61, "square range");
+    CHECK(eq61(a, add(v0, v1)) && eq61(b, conj(sub(v0, v1))), "square_pair 61");
+    const e31 yk = rnd31(), ymk = rnd31(), t3 = rnd31();
+    e31 p = yk, q = ymk;
+    square_pair(p, q, add(e31{1, 0}, t3));
+    const e31 c3 = conj(ymk), w0 = add(yk, c3), w1 = sub(yk, c3);
+    const e31 x0 = sub(sqr(w0), mul(sqr(w1), t3)), x1 = mul(w0, add(w1, w1));
+    CHECK(eq31(p, add(x0, x1)) && eq31(q, conj(sub(x0, x1))), "square_pair 31");
+  }
+  printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
+  return fails ? 1 : 0;
+}

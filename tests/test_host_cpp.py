@@ -14,3 +14,14 @@ def test_host_plan_and_residue_code(tmp_path):
                     "-o", exe], check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:]
+
+
+def test_transform_arithmetic(tmp_path):
+    """Split-twiddle products, lazy butterflies and the centre squaring formula
+    (tests/cpp/test_arith.cpp) against plain modular arithmetic."""
+    csrc = os.path.join(ROOT, "mersenne2gpu_amd", "csrc")
+    exe = str(tmp_path / "test_arith")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", csrc, os.path.join(ROOT, "tests", "cpp", "test_arith.cpp"),
+                    "-o", exe], check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:]
