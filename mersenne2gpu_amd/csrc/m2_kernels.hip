@@ -194,15 +194,38 @@ __device__ __forceinline__ int stage_pass_twiddles(e61* T61, e31* T31, const Pas
 // independent transform. Step `lam` (relative level) with radix r: the
 // butterflies combine rows blk*(R>>lam) + off + e*d, e < r, d = R >> (lam +
 // log2 r); the twist tau^e of block blk is at Ts[b * span + blk*(r-1) + e-1].
-template <bool FWD, int RADIX>
+// Thread roles. With SPLIT (the 512-thread multi-pass kernels) the first
+// half of the workgroup works on the M61 field and the second half on the
+// M31 field, so each SIMD holds one wave of each and runs one while the other
+// waits on LDS; without SPLIT every thread does both fields.
+template <bool SPLIT>
+struct Lanes {
+  int tid, n;
+  bool f61, f31;
+  __device__ __forceinline__ Lanes() {
+    if (SPLIT) {
+      n = (int)blockDim.x >> 1;
+      f61 = (int)threadIdx.x < n;
+      f31 = !f61;
+      tid = f61 ? (int)threadIdx.x : (int)threadIdx.x - n;
+    } else {
+      n = (int)blockDim.x;
+      f61 = f31 = true;
+      tid = (int)threadIdx.x;
+    }
+  }
+};
+
+template <bool FWD, int RADIX, bool SPLIT>
 __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW, int nb, int lam, const e61* Ts61,
                                          const e31* Ts31) {
   constexpr int LR = RADIX == 8 ? 3 : (RADIX == 4 ? 2 : 1);
+  const Lanes<SPLIT> ln;
   const int logd = logR - lam - LR;
   const int logU = logR - LR + logW;  // butterflies per block (log2)
   const int total = nb << logU;
   const int span = (1 << lam) * (RADIX - 1);
-  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+  for (int t = ln.tid; t < total; t += ln.n) {
     const int b = t >> logU;
     const int u = t & ((1 << logU) - 1);
     const int col = u & ((1 << logW) - 1);
@@ -212,38 +235,48 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
     const int row0 = (blk << (logR - lam)) + off;
     const int base = (b << (logR + logW)) + (row0 << logW) + col;
     const int stride = 1 << (logd + logW);
-    const e61* T61 = Ts61 + b * span + blk * (RADIX - 1);
-    const e31* T31 = Ts31 + b * span + blk * (RADIX - 1);
-    e61 v[RADIX];
-    e31 x[RADIX];
+    const int tw = b * span + blk * (RADIX - 1);
+    if (ln.f61) {
+      const e61* T61 = Ts61 + tw;
+      e61 v[RADIX];
 #pragma unroll
-    for (int e = 0; e < RADIX; ++e) {
-      v[e] = L61[PADX(base + e * stride)];
-      x[e] = L31[PADX(base + e * stride)];
-    }
-    if (FWD) {
+      for (int e = 0; e < RADIX; ++e) v[e] = L61[PADX(base + e * stride)];
+      if (FWD) {
 #pragma unroll
-      for (int e = 1; e < RADIX; ++e) {
-        v[e] = mul_tw(v[e], T61[e - 1]);
-        x[e] = mul_tw(x[e], T31[e - 1]);
+        for (int e = 1; e < RADIX; ++e) v[e] = mul_tw(v[e], T61[e - 1]);
+        if (RADIX == 8) dft8_fwd_lazy(v);
+        if (RADIX == 4) dft4_fwd_lazy(v);
+        if (RADIX == 2) dft2_fwd_lazy(v[0], v[1]);
+      } else {
+        if (RADIX == 8) dft8_inv_lazy(v);
+        if (RADIX == 4) dft4_inv_lazy(v);
+        if (RADIX == 2) dft2_inv_lazy(v[0], v[1]);
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) v[e] = mulconj_tw(v[e], T61[e - 1]);
       }
-      if (RADIX == 8) { dft8_fwd_lazy(v); dft8_fwd(x); }
-      if (RADIX == 4) { dft4_fwd_lazy(v); dft4_fwd(x); }
-      if (RADIX == 2) { dft2_fwd_lazy(v[0], v[1]); dft2(x[0], x[1]); }
-    } else {
-      if (RADIX == 8) { dft8_inv_lazy(v); dft8_inv(x); }
-      if (RADIX == 4) { dft4_inv_lazy(v); dft4_inv(x); }
-      if (RADIX == 2) { dft2_inv_lazy(v[0], v[1]); dft2(x[0], x[1]); }
 #pragma unroll
-      for (int e = 1; e < RADIX; ++e) {
-        v[e] = mulconj_tw(v[e], T61[e - 1]);
-        x[e] = mulconj_tw(x[e], T31[e - 1]);
-      }
+      for (int e = 0; e < RADIX; ++e) L61[PADX(base + e * stride)] = v[e];
     }
+    if (ln.f31) {
+      const e31* T31 = Ts31 + tw;
+      e31 x[RADIX];
 #pragma unroll
-    for (int e = 0; e < RADIX; ++e) {
-      L61[PADX(base + e * stride)] = v[e];
-      L31[PADX(base + e * stride)] = x[e];
+      for (int e = 0; e < RADIX; ++e) x[e] = L31[PADX(base + e * stride)];
+      if (FWD) {
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) x[e] = mul_tw(x[e], T31[e - 1]);
+        if (RADIX == 8) dft8_fwd(x);
+        if (RADIX == 4) dft4_fwd(x);
+        if (RADIX == 2) dft2(x[0], x[1]);
+      } else {
+        if (RADIX == 8) dft8_inv(x);
+        if (RADIX == 4) dft4_inv(x);
+        if (RADIX == 2) dft2(x[0], x[1]);
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) x[e] = mulconj_tw(x[e], T31[e - 1]);
+      }
+#pragma unroll
+      for (int e = 0; e < RADIX; ++e) L31[PADX(base + e * stride)] = x[e];
     }
   }
   __syncthreads();
@@ -251,7 +284,7 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
 
 // All steps of a pass, forward (in order) or inverse (reversed), twiddles in
 // LDS as laid out by stage_pass_twiddles.
-template <bool FWD>
+template <bool FWD, bool SPLIT>
 __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, const Pass& ps, const e61* T61,
                                          const e31* T31) {
   int offs[MAX_STEPS];
@@ -264,9 +297,9 @@ __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, c
     const int si = FWD ? k : ps.nsteps - 1 - k;
     const Step st = ps.steps[si];
     const int lam = st.ell - ps.ell0;
-    if (st.radix == 8) lds_step<FWD, 8>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
-    else if (st.radix == 4) lds_step<FWD, 4>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
-    else lds_step<FWD, 2>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
+    if (st.radix == 8) lds_step<FWD, 8, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
+    else if (st.radix == 4) lds_step<FWD, 4, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
+    else lds_step<FWD, 2, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
   }
 }
 
@@ -274,38 +307,46 @@ __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, c
 // mirror formula, k = 0 special) at LDS offset 0, and, when nself = 2, block 1
 // (positions Cb..2Cb-1, k pairs with 3Cb-1-k) at LDS offset Cb. S61/S31 hold
 // the centre twiddles of the even positions: block 0's Cb/2, then block 1's.
+template <bool SPLIT>
 __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int nself, const e61* S61,
                                             const e31* S31) {
+  const Lanes<SPLIT> ln;
   const int Cb = 1 << logCb;
   const int half = Cb >> 1;
-  for (int q = threadIdx.x; q < nself * half; q += blockDim.x) {
+  for (int q = ln.tid; q < nself * half; q += ln.n) {
     const int blk = q >= half ? 1 : 0;
     const int j = q - blk * half;
     const int k = 2 * j;
     const int kidx = blk * Cb + k;
     const int mkidx = blk ? Cb + (Cb - 1 - k) : (k ? (3 << (31 - __clz(k))) - k - 1 : 0);
-    const e61 t61 = S61[q];
-    const e31 t31 = S31[q];
     if (blk == 0 && k == 0) {
-      e61 a = L61[PADX(0)], a2 = a;
-      e31 b = L31[PADX(0)], b2 = b;
-      square_pair(a, a2, t61);
-      square_pair(b, b2, t31);
-      const e61 z1 = red(L61[PADX(1)]);
-      const e31 y1 = L31[PADX(1)];
-      L61[PADX(0)] = a;
-      L31[PADX(0)] = b;
-      L61[PADX(1)] = sqr(add(z1, z1));
-      L31[PADX(1)] = sqr(add(y1, y1));
+      if (ln.f61) {
+        e61 a = L61[PADX(0)], a2 = a;
+        square_pair(a, a2, S61[q]);
+        const e61 z1 = red(L61[PADX(1)]);
+        L61[PADX(0)] = a;
+        L61[PADX(1)] = sqr(add(z1, z1));
+      }
+      if (ln.f31) {
+        e31 b = L31[PADX(0)], b2 = b;
+        square_pair(b, b2, S31[q]);
+        const e31 y1 = L31[PADX(1)];
+        L31[PADX(0)] = b;
+        L31[PADX(1)] = sqr(add(y1, y1));
+      }
     } else {
-      e61 a = L61[PADX(kidx)], c = L61[PADX(mkidx)];
-      e31 b = L31[PADX(kidx)], d = L31[PADX(mkidx)];
-      square_pair(a, c, t61);
-      square_pair(b, d, t31);
-      L61[PADX(kidx)] = a;
-      L61[PADX(mkidx)] = c;
-      L31[PADX(kidx)] = b;
-      L31[PADX(mkidx)] = d;
+      if (ln.f61) {
+        e61 a = L61[PADX(kidx)], c = L61[PADX(mkidx)];
+        square_pair(a, c, S61[q]);
+        L61[PADX(kidx)] = a;
+        L61[PADX(mkidx)] = c;
+      }
+      if (ln.f31) {
+        e31 b = L31[PADX(kidx)], d = L31[PADX(mkidx)];
+        square_pair(b, d, S31[q]);
+        L31[PADX(kidx)] = b;
+        L31[PADX(mkidx)] = d;
+      }
     }
   }
   __syncthreads();
@@ -314,9 +355,11 @@ __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int n
 // Squaring across two LDS blocks B (at 0) and B2 = 3*2^a - 1 - B (at Cb):
 // position B*Cb + o pairs with B2*Cb + (Cb - 1 - o). S61: centre twiddles of
 // block B's even positions (Cb/2) followed by block B2's (Cb/2).
+template <bool SPLIT>
 __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, const e61* S61, const e31* S31) {
+  const Lanes<SPLIT> ln;
   const int Cb = 1 << logCb;
-  for (int o = threadIdx.x; o < Cb; o += blockDim.x) {
+  for (int o = ln.tid; o < Cb; o += ln.n) {
     int kidx, mkidx, ti;
     if ((o & 1) == 0) {
       kidx = o;
@@ -327,14 +370,18 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, cons
       mkidx = o;
       ti = (Cb >> 1) + ((Cb - 1 - o) >> 1);
     }
-    e61 a = L61[PADX(kidx)], c = L61[PADX(mkidx)];
-    e31 b = L31[PADX(kidx)], d = L31[PADX(mkidx)];
-    square_pair(a, c, S61[ti]);
-    square_pair(b, d, S31[ti]);
-    L61[PADX(kidx)] = a;
-    L61[PADX(mkidx)] = c;
-    L31[PADX(kidx)] = b;
-    L31[PADX(mkidx)] = d;
+    if (ln.f61) {
+      e61 a = L61[PADX(kidx)], c = L61[PADX(mkidx)];
+      square_pair(a, c, S61[ti]);
+      L61[PADX(kidx)] = a;
+      L61[PADX(mkidx)] = c;
+    }
+    if (ln.f31) {
+      e31 b = L31[PADX(kidx)], d = L31[PADX(mkidx)];
+      square_pair(b, d, S31[ti]);
+      L31[PADX(kidx)] = b;
+      L31[PADX(mkidx)] = d;
+    }
   }
   __syncthreads();
 }
@@ -345,17 +392,15 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, cons
 template <int B, class Src, class Dst>
 __device__ __forceinline__ void gather2(e61* L61, e31* L31, const e61* __restrict__ g61, const e31* __restrict__ g31,
                                         int count, Src src, Dst dst) {
-  // Full batches have no per-lane conditions, so the staged values stay in
-  // registers; the remainder (count not a multiple of B * blockDim) goes one
-  // element at a time.
+  // Loads are unconditional (lanes past the end re-read the last element) so
+  // that the staged values stay in registers; only the LDS stores are masked.
   const int step = B * (int)blockDim.x;
-  int base = 0;
-  for (; base + step <= count; base += step) {
+  for (int base = 0; base < count; base += step) {
     uint64_t a[B], b[B];
     uint32_t c[B], d[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-      const uint32_t j = src(base + k * (int)blockDim.x + (int)threadIdx.x);
+      const uint32_t j = src(min(base + k * (int)blockDim.x + (int)threadIdx.x, count - 1));
       a[k] = g61[j].a;
       b[k] = g61[j].b;
       c[k] = g31[j].a;
@@ -363,16 +408,33 @@ __device__ __forceinline__ void gather2(e61* L61, e31* L31, const e61* __restric
     }
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-      const int j = dst(base + k * (int)blockDim.x + (int)threadIdx.x);
-      L61[j] = e61{a[k], b[k]};
-      L31[j] = e31{c[k], d[k]};
+      const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
+      if (i < count) {
+        const int j = dst(i);
+        L61[j] = e61{a[k], b[k]};
+        L31[j] = e31{c[k], d[k]};
+      }
     }
   }
-  for (int i = base + (int)threadIdx.x; i < count; i += blockDim.x) {
-    const uint32_t j = src(i);
-    const int t = dst(i);
-    L61[t] = g61[j];
-    L31[t] = g31[j];
+}
+
+// The same for one e61 array.
+template <int B, class Src, class Dst>
+__device__ __forceinline__ void gather1(e61* L61, const e61* __restrict__ g61, int count, Src src, Dst dst) {
+  const int step = B * (int)blockDim.x;
+  for (int base = 0; base < count; base += step) {
+    uint64_t a[B], b[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const uint32_t j = src(min(base + k * (int)blockDim.x + (int)threadIdx.x, count - 1));
+      a[k] = g61[j].a;
+      b[k] = g61[j].b;
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
+      if (i < count) L61[dst(i)] = e61{a[k], b[k]};
+    }
   }
 }
 
@@ -411,9 +473,9 @@ __global__ void __launch_bounds__(256) k_small(const TestDesc* __restrict__ test
   for (uint32_t it = 0; it < td.iters; ++it) {
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_weight(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D);
     __syncthreads();
-    lds_pass<true>(L61, L31, 0, 1, ps, T61, T31);
-    square_self(L61, L31, logH, 1, S61, S31);
-    lds_pass<false>(L61, L31, 0, 1, ps, T61, T31);
+    lds_pass<true, false>(L61, L31, 0, 1, ps, T61, T31);
+    square_self<false>(L61, L31, logH, 1, S61, S31);
+    lds_pass<false, false>(L61, L31, 0, 1, ps, T61, T31);
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
       const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
       store_carry(gs, s, seg_normalise(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D, c0));
@@ -439,9 +501,12 @@ __device__ __forceinline__ uint32_t prev_strip_carry(uint32_t cg, uint32_t r, ui
   return (NS - 1) * R + (r ? r - 1 : R - 1);
 }
 
+// Threads per workgroup of the multi-pass kernels: 256 per field.
+constexpr int MP_THREADS = 512;
+
 constexpr int OUTER_TW = PASS_MAX_ELEMS / 2;  // R - 1 twiddles per block, R <= 512 here
 
-__global__ void __launch_bounds__(256) k_outer_fwd(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+__global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                                    const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
@@ -458,27 +523,11 @@ __global__ void __launch_bounds__(256) k_outer_fwd(const TestDesc* __restrict__ 
   const uint32_t cg = blockIdx.x, NS = (1u << logS) >> logW;
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
-  // digits of the strip are contiguous; with them, the carry entering each
-  // strip row (loads batched: up to 8 digit pairs and 2 carries per thread
-  // in flight)
+  // digits of the strip are contiguous (loads batched), then the carry
+  // entering each strip row
   const e61* src = zd + (size_t)cg * (R * W);
-  {
-    const int n = R * W, step = 8 * (int)blockDim.x;
-    int base = 0;
-    for (; base + step <= n; base += step) {
-      uint64_t a[8], b[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
-        a[k] = src[i].a;
-        b[k] = src[i].b;
-      }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) L61[PADX(base + k * (int)blockDim.x + (int)threadIdx.x)] = e61{a[k], b[k]};
-    }
-    for (int i = base + (int)threadIdx.x; i < n; i += blockDim.x) L61[PADX(i)] = src[i];
-    for (int rr = threadIdx.x; rr < R; rr += blockDim.x) gcin[rr] = G[prev_strip_carry(cg, rr, R, NS)];
-  }
+  gather1<4>(L61, src, R * W, [](int i) { return (uint32_t)i; }, [](int i) { return PADX(i); });
+  for (int rr = threadIdx.x; rr < R; rr += blockDim.x) gcin[rr] = G[prev_strip_carry(cg, rr, R, NS)];
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   // prologue: carry-in of each strip-row, then weight
@@ -491,7 +540,7 @@ __global__ void __launch_bounds__(256) k_outer_fwd(const TestDesc* __restrict__ 
     seg_weight(L61, L31, idx0, 2u * e0, D);
   }
   __syncthreads();
-  lds_pass<true>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<true, true>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
     z61[e] = L61[PADX(i)];
@@ -499,7 +548,7 @@ __global__ void __launch_bounds__(256) k_outer_fwd(const TestDesc* __restrict__ 
   }
 }
 
-__global__ void __launch_bounds__(256) k_outer_inv(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+__global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                                    const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
@@ -517,12 +566,12 @@ __global__ void __launch_bounds__(256) k_outer_inv(const TestDesc* __restrict__ 
   const uint32_t cg = blockIdx.x;
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
-  gather2<8>(L61, L31, z61, z31, R * W,
+  gather2<4>(L61, L31, z61, z31, R * W,
              [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); },
              [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
-  lds_pass<false>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<false, true>(L61, L31, logW, 1, ps, T61, T31);
   const int subs = W / SEG_ELEMS;
   for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
     const int r = it / subs, v = it - r * subs;
@@ -547,7 +596,7 @@ __global__ void __launch_bounds__(256) k_outer_inv(const TestDesc* __restrict__ 
 // Middle pass: levels [ell0, ell0 + A) with no carry work. Level-ell0 block
 // j (of R * S elements) is split into S/W column groups.
 template <bool FWD>
-__global__ void __launch_bounds__(256) k_mid(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+__global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                              const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
@@ -561,14 +610,12 @@ __global__ void __launch_bounds__(256) k_mid(const TestDesc* __restrict__ tests,
   const uint32_t jb = blockIdx.x / NS, cg = blockIdx.x - jb * NS;
   const uint32_t base = (jb << (ps.A + logS)) + (cg << logW);
   const uint32_t j0[1] = {jb};
-  for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
-    const uint32_t e = base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1));
-    L61[PADX(i)] = z61[e];
-    L31[PADX(i)] = z31[e];
-  }
+  gather2<4>(L61, L31, z61, z31, R * W,
+             [&](int i) { return base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1)); },
+             [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
-  lds_pass<FWD>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<FWD, true>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1));
     z61[e] = L61[PADX(i)];
@@ -580,7 +627,7 @@ __global__ void __launch_bounds__(256) k_mid(const TestDesc* __restrict__ tests,
 // C = 2^logC elements that the real-data pairing links; workgroup 0 takes
 // blocks 0 and 1, which pair with themselves. Forward levels [ps.ell0, L),
 // square, inverse. There are h / (2C) workgroups.
-__global__ void __launch_bounds__(256) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
+__global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
                                                 const e31* __restrict__ ctw31, uint32_t crec, Pass ps) {
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
@@ -602,19 +649,19 @@ __global__ void __launch_bounds__(256) k_centre(const TestDesc* __restrict__ tes
   }
   // the blocks' elements, then this workgroup's twiddle record (step
   // twiddles, then squaring twiddles), all loads batched
-  gather2<8>(L61, L31, z61, z31, nb << logC,
+  gather2<4>(L61, L31, z61, z31, nb << logC,
              [&](int i) { return ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1)); },
              [](int i) { return PADX(i); });
   const int nT = (int)crec - C;
   const e61* r61 = ctw61 + (size_t)g * crec;
   const e31* r31 = ctw31 + (size_t)g * crec;
-  gather2<8>(T61, T31, r61, r31, nT, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
-  gather2<4>(S61, S31, r61 + nT, r31 + nT, C, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+  gather2<4>(T61, T31, r61, r31, nT, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+  gather2<2>(S61, S31, r61 + nT, r31 + nT, C, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
   __syncthreads();
-  lds_pass<true>(L61, L31, 0, nb, ps, T61, T31);
-  if (g == 0) square_self(L61, L31, logC, 2, S61, S31);
-  else square_cross(L61, L31, logC, S61, S31);
-  lds_pass<false>(L61, L31, 0, nb, ps, T61, T31);
+  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31);
+  if (g == 0) square_self<true>(L61, L31, logC, 2, S61, S31);
+  else square_cross<true>(L61, L31, logC, S61, S31);
+  lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31);
   for (int i = threadIdx.x; i < (nb << logC); i += blockDim.x) {
     const uint32_t e = ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1));
     z61[e] = L61[PADX(i)];
@@ -656,7 +703,7 @@ hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& l
 
 hipError_t launch_iteration(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
   return iteration(P, tests, ntests, [&](const char*, auto kernel, dim3 grid, auto... args) -> hipError_t {
-    hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, args...);
+    hipLaunchKernelGGL(kernel, grid, dim3(MP_THREADS), 0, st, args...);
     return hipGetLastError();
   });
 }
@@ -668,7 +715,7 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
   hipError_t e = iteration(P, tests, ntests, [&](const char* name, auto kernel, dim3 grid, auto... args) -> hipError_t {
     if (k >= max_events) return hipErrorInvalidValue;
     names[k] = name;
-    hipExtLaunchKernelGGL(kernel, grid, dim3(256), 0, st, starts[k], stops[k], 0, args...);
+    hipExtLaunchKernelGGL(kernel, grid, dim3(MP_THREADS), 0, st, starts[k], stops[k], 0, args...);
     ++k;
     return hipGetLastError();
   });
