@@ -504,6 +504,18 @@ __device__ __forceinline__ uint32_t prev_strip_carry(uint32_t cg, uint32_t r, ui
 // Threads per workgroup of the multi-pass kernels: 256 per field.
 constexpr int MP_THREADS = 512;
 
+// Workgroups are dealt to the 8 XCDs round-robin by their linear index, and
+// each XCD has its own L2. Neighbouring column strips share cache lines (a
+// strip row is 64 bytes of M61 and 32 of M31 data), so strip c is given to
+// workgroup index b with c = (b mod 8) (n/8) + b/8: each XCD then works on
+// one contiguous range of strips and whole lines stay in one L2. (Needs n a
+// multiple of 8; gridDim.x is n, so the XCD of a workgroup does not depend
+// on blockIdx.y.)
+constexpr uint32_t NUM_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_strip(uint32_t b, uint32_t n) {
+  return (n % NUM_XCD) ? b : (b % NUM_XCD) * (n / NUM_XCD) + b / NUM_XCD;
+}
+
 constexpr int OUTER_TW = PASS_MAX_ELEMS / 2;  // R - 1 twiddles per block, R <= 512 here
 
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
@@ -520,7 +532,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   const Carry128* __restrict__ G = td.G;
   const DigitGen D = td.dgen;
   const int R = 1 << ps.A, W = 1 << logW;
-  const uint32_t cg = blockIdx.x, NS = (1u << logS) >> logW;
+  const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
   // digits of the strip are contiguous (loads batched), then the carry
@@ -563,7 +575,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   const DigitGen D = td.dgen;
   const int mode = td.mode;
   const int R = 1 << ps.A, W = 1 << logW;
-  const uint32_t cg = blockIdx.x;
+  const uint32_t cg = xcd_strip(blockIdx.x, (1u << logS) >> logW);
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
   gather2<4>(L61, L31, z61, z31, R * W,
@@ -607,7 +619,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
   e31* __restrict__ z31 = td.z31;
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW;
-  const uint32_t jb = blockIdx.x / NS, cg = blockIdx.x - jb * NS;
+  const uint32_t bx = xcd_strip(blockIdx.x, gridDim.x);
+  const uint32_t jb = bx / NS, cg = bx - jb * NS;
   const uint32_t base = (jb << (ps.A + logS)) + (cg << logW);
   const uint32_t j0[1] = {jb};
   gather2<4>(L61, L31, z61, z31, R * W,
