@@ -71,6 +71,7 @@ def main(argv=None) -> int:
     ap.add_argument("--count", type=int, default=512)
     ap.add_argument("--full", action="store_true", help="run every test to completion")
     ap.add_argument("--cap", type=int, default=2000, help="iterations per test when not --full")
+    ap.add_argument("--chunk", type=int, default=50000, help="iterations per test between progress lines")
     args = ap.parse_args(argv)
 
     ps = W.batch_exponents(args.seed, args.count)
@@ -86,9 +87,20 @@ def main(argv=None) -> int:
     t0 = time.perf_counter()
     b = m2.Batch(ctx, ps, "prp", iters=stops)
     t_create = time.perf_counter() - t0
+    # advanced in chunks of --chunk iterations per test, with a progress line
+    # after each (the device stays busy: every chunk is enqueued before the wait)
     t0 = time.perf_counter()
-    b.run(0)
-    b.sync()
+    done_iters = 0
+    while True:
+        b.run(args.chunk)
+        b.sync()
+        done = sum(min(b.iteration(i), stops[i]) for i in range(len(ps)))
+        finished = sum(b.iteration(i) >= stops[i] for i in range(len(ps)))
+        log("  %.1f s: %d/%d tests finished, %.1f%% of iterations" % (
+            time.perf_counter() - t0, finished, len(ps), 100.0 * done / sum(stops)))
+        if done == done_iters or finished == len(ps):
+            break
+        done_iters = done
     t_run = time.perf_counter() - t0
     log("batched: create %.3f s, run %.3f s" % (t_create, t_run))
 

@@ -31,10 +31,6 @@ namespace m2 {
 
 typedef __int128 i128;
 
-// M61*M31, and the point above which a CRT value is read as negative (LL's -2
-// can make a digit -1, and convolution terms slightly negative).
-#define M2_CRT_M (((i128)P61) * (i128)P31)
-#define M2_CRT_NEG (((i128)3) << 90)
 
 __device__ __forceinline__ int64_t to_i64(uint64_t x) { return (int64_t)x; }
 
@@ -66,15 +62,27 @@ __device__ __forceinline__ i128 crt_digit(uint64_t s, uint32_t t, uint32_t w61, 
   t = a31 ? rot31(t, 31u - a31) : t;
   uint64_t u = sub61(s, t);
   u = add61(u, rot61(u, 31));
-  i128 x = (((i128)u) << 31) - (i128)u + (i128)t;
-  if (x >= M2_CRT_NEG) x -= M2_CRT_M;
-  return x;
+  // x = u (2^31 - 1) + t in [0, M), M = P61 P31 = P61 (2^31 - 1). Values at or
+  // above about 3 2^90 stand for x - M = (u - P61)(2^31 - 1) + t. Deciding on
+  // u >= 3 2^59 instead of x >= 3 2^90 differs only for x within 3 2^59 below
+  // 3 2^90, where no value lies: convolution terms are below 2^91 and the
+  // negative ones (from LL's -2) are tiny.
+  const int64_t v = (int64_t)u - ((u >= (3ull << 59)) ? (int64_t)P61 : 0);
+  return ((i128)v << 31) - (i128)v + (i128)t;
 }
 
+// acc >> w (arithmetic) for 0 <= w < 64, as two 64-bit shifts of the words.
+__device__ __forceinline__ i128 shr_w(i128 acc, uint32_t w) {
+  const uint64_t lo = (uint64_t)acc;
+  const int64_t hi = (int64_t)(acc >> 64);
+  const uint64_t clo = (lo >> w) | (((uint64_t)hi << 1) << (63u - w));
+  return (i128)((unsigned __int128)(uint64_t)(hi >> w) << 64 | clo);
+}
+
+// Digit of width w (< 64) and the carry out of acc.
 __device__ __forceinline__ int64_t norm_digit(i128 acc, uint32_t w, i128& carry) {
-  const uint64_t mask = w ? ((~0ull) >> (64u - w)) : 0ull;
-  carry = acc >> w;
-  return (int64_t)((uint64_t)acc & mask);
+  carry = shr_w(acc, w);
+  return (int64_t)((uint64_t)acc & ((1ull << w) - 1ull));
 }
 
 __device__ __forceinline__ i128 load_carry(const Carry128* G, uint32_t i) {
@@ -286,14 +294,14 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
 // LDS as laid out by stage_pass_twiddles.
 template <bool FWD, bool SPLIT>
 __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, const Pass& ps, const e61* T61,
-                                         const e31* T31) {
+                                         const e31* T31, int k0 = 0, int k1 = MAX_STEPS) {
   int offs[MAX_STEPS];
   int lo = 0;
   for (int si = 0; si < ps.nsteps; ++si) {
     offs[si] = lo;
     lo += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
   }
-  for (int k = 0; k < ps.nsteps; ++k) {
+  for (int k = k0; k < min(k1, ps.nsteps); ++k) {
     const int si = FWD ? k : ps.nsteps - 1 - k;
     const Step st = ps.steps[si];
     const int lam = st.ell - ps.ell0;
@@ -660,18 +668,46 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
     B = (1u << a) + (g - (1u << (a - 1)));
     B2 = 3u * (1u << a) - 1u - B;
   }
-  // the blocks' elements, then this workgroup's twiddle record (step
-  // twiddles, then squaring twiddles), all loads batched
+  // The blocks' elements and the twiddles of all but the last forward step go
+  // to LDS first; the rest of this workgroup's twiddle record (the last
+  // step's twists, then the squaring twiddles: at most CPF per thread) is
+  // loaded into registers now and written to LDS after the other steps, so
+  // its loads overlap their arithmetic.
   gather2<4>(L61, L31, z61, z31, nb << logC,
              [&](int i) { return ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1)); },
              [](int i) { return PADX(i); });
   const int nT = (int)crec - C;
+  int lo_last = 0;
+  for (int si = 0; si + 1 < ps.nsteps; ++si) lo_last += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
   const e61* r61 = ctw61 + (size_t)g * crec;
   const e31* r31 = ctw31 + (size_t)g * crec;
-  gather2<4>(T61, T31, r61, r31, nT, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
-  gather2<2>(S61, S31, r61 + nT, r31 + nT, C, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+  gather2<1>(T61, T31, r61, r31, lo_last, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+  constexpr int CPF = 6;  // (2 (2^c - 1) + 2^c - lo_last) <= 3 2^10 = CPF * 512
+  uint64_t pa[CPF], pb[CPF];
+  uint32_t pc[CPF], pd[CPF];
+#pragma unroll
+  for (int k = 0; k < CPF; ++k) {
+    const int i = min(lo_last + k * (int)blockDim.x + (int)threadIdx.x, (int)crec - 1);
+    pa[k] = r61[i].a;
+    pb[k] = r61[i].b;
+    pc[k] = r31[i].a;
+    pd[k] = r31[i].b;
+  }
   __syncthreads();
-  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31);
+  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, 0, ps.nsteps - 1);
+#pragma unroll
+  for (int k = 0; k < CPF; ++k) {
+    const int i = lo_last + k * (int)blockDim.x + (int)threadIdx.x;
+    if (i < nT) {
+      T61[i] = e61{pa[k], pb[k]};
+      T31[i] = e31{pc[k], pd[k]};
+    } else if (i < (int)crec) {
+      S61[i - nT] = e61{pa[k], pb[k]};
+      S31[i - nT] = e31{pc[k], pd[k]};
+    }
+  }
+  __syncthreads();
+  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, ps.nsteps - 1, ps.nsteps);
   if (g == 0) square_self<true>(L61, L31, logC, 2, S61, S31);
   else square_cross<true>(L61, L31, logC, S61, S31);
   lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31);
