@@ -85,11 +85,33 @@ __device__ __forceinline__ int64_t norm_digit(i128 acc, uint32_t w, i128& carry)
   return (int64_t)((uint64_t)acc & ((1ull << w) - 1ull));
 }
 
-__device__ __forceinline__ i128 load_carry(const Carry128* G, uint32_t i) {
+// Per-test buffers reach the kernels through the test descriptors, i.e. as
+// pointers loaded from memory, which the compiler treats as generic (flat)
+// addresses. They are device (global) memory: casting them to the global
+// address space gives global loads and stores, which do not wait on the LDS
+// counter as flat ones do.
+#define M2_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ M2_GLOBAL T* gptr(T* p) {
+  return (M2_GLOBAL T*)p;
+}
+
+// Element copies between global memory and registers/LDS (structure
+// assignment does not cross address spaces).
+__device__ __forceinline__ e61 gld(const M2_GLOBAL e61* p) { return e61{p->a, p->b}; }
+__device__ __forceinline__ e31 gld(const M2_GLOBAL e31* p) { return e31{p->a, p->b}; }
+__device__ __forceinline__ Carry128 gld(const M2_GLOBAL Carry128* p) { return Carry128{p->lo, p->hi}; }
+__device__ __forceinline__ void gst(M2_GLOBAL e61* p, e61 v) { p->a = v.a; p->b = v.b; }
+__device__ __forceinline__ void gst(M2_GLOBAL e31* p, e31 v) { p->a = v.a; p->b = v.b; }
+__device__ __forceinline__ void gst(M2_GLOBAL Carry128* p, Carry128 v) { p->lo = v.lo; p->hi = v.hi; }
+
+template <class CP>
+__device__ __forceinline__ i128 load_carry(CP G, uint32_t i) {
   const Carry128 c = G[i];
   return (((i128)c.hi) << 64) | (i128)c.lo;
 }
-__device__ __forceinline__ void store_carry(Carry128* G, uint32_t i, i128 v) {
+template <class CP>
+__device__ __forceinline__ void store_carry(CP G, uint32_t i, i128 v) {
   G[i].lo = (uint64_t)v;
   G[i].hi = (int64_t)(v >> 64);
 }
@@ -163,9 +185,8 @@ __device__ __forceinline__ void seg_weight(e61* L61, e31* L31, int idx0, uint32_
 // LDS at offset lo + b * (1 << lam) * (r-1), lo = sum of the earlier steps.
 __device__ __forceinline__ int twiddle_span(const Step& st, int lam) { return (1 << lam) * (st.radix - 1); }
 
-template <int B, class Src, class Dst>
-__device__ __forceinline__ void gather2(e61* L61, e31* L31, const e61* __restrict__ g61, const e31* __restrict__ g31,
-                                        int count, Src src, Dst dst);
+template <int B, class G61, class G31, class Src, class Dst>
+__device__ __forceinline__ void gather2(e61* L61, e31* L31, G61 g61, G31 g31, int count, Src src, Dst dst);
 
 __device__ __forceinline__ int stage_pass_twiddles(e61* T61, e31* T31, const Pass& ps, int nb, const uint32_t* j0,
                                                    const e61* __restrict__ tw61, const e31* __restrict__ tw31) {
@@ -397,51 +418,69 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, cons
 // Copies count elements of both fields, global index src(i) -> LDS index
 // dst(i), with each thread issuing B loads before its LDS stores so that the
 // loads' latencies overlap instead of adding up.
-template <int B, class Src, class Dst>
-__device__ __forceinline__ void gather2(e61* L61, e31* L31, const e61* __restrict__ g61, const e31* __restrict__ g31,
-                                        int count, Src src, Dst dst) {
-  // Loads are unconditional (lanes past the end re-read the last element) so
-  // that the staged values stay in registers; only the LDS stores are masked.
+template <int B, class G61, class G31, class Src, class Dst>
+__device__ __forceinline__ void gather2(e61* L61, e31* L31, G61 g61, G31 g31, int count, Src src, Dst dst) {
+  // Loads are unconditional (in the last, partial batch lanes past the end
+  // re-read the last element) so that the staged values stay in registers;
+  // full batches store unmasked.
   const int step = B * (int)blockDim.x;
   for (int base = 0; base < count; base += step) {
+    const bool full = base + step <= count;
     uint64_t a[B], b[B];
     uint32_t c[B], d[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-      const uint32_t j = src(min(base + k * (int)blockDim.x + (int)threadIdx.x, count - 1));
+      const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
+      const uint32_t j = src(full ? i : min(i, count - 1));
       a[k] = g61[j].a;
       b[k] = g61[j].b;
       c[k] = g31[j].a;
       d[k] = g31[j].b;
     }
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
-      if (i < count) {
-        const int j = dst(i);
+      for (int k = 0; k < B; ++k) {
+        const int j = dst(base + k * (int)blockDim.x + (int)threadIdx.x);
         L61[j] = e61{a[k], b[k]};
         L31[j] = e31{c[k], d[k]};
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
+        if (i < count) {
+          const int j = dst(i);
+          L61[j] = e61{a[k], b[k]};
+          L31[j] = e31{c[k], d[k]};
+        }
       }
     }
   }
 }
 
 // The same for one e61 array.
-template <int B, class Src, class Dst>
-__device__ __forceinline__ void gather1(e61* L61, const e61* __restrict__ g61, int count, Src src, Dst dst) {
+template <int B, class G61, class Src, class Dst>
+__device__ __forceinline__ void gather1(e61* L61, G61 g61, int count, Src src, Dst dst) {
   const int step = B * (int)blockDim.x;
   for (int base = 0; base < count; base += step) {
+    const bool full = base + step <= count;
     uint64_t a[B], b[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-      const uint32_t j = src(min(base + k * (int)blockDim.x + (int)threadIdx.x, count - 1));
+      const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
+      const uint32_t j = src(full ? i : min(i, count - 1));
       a[k] = g61[j].a;
       b[k] = g61[j].b;
     }
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
-      if (i < count) L61[dst(i)] = e61{a[k], b[k]};
+      for (int k = 0; k < B; ++k) L61[dst(base + k * (int)blockDim.x + (int)threadIdx.x)] = e61{a[k], b[k]};
+    } else {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const int i = base + k * (int)blockDim.x + (int)threadIdx.x;
+        if (i < count) L61[dst(i)] = e61{a[k], b[k]};
+      }
     }
   }
 }
@@ -468,7 +507,7 @@ __global__ void __launch_bounds__(256) k_small(const TestDesc* __restrict__ test
   __shared__ e31 S31[1 << (SMALL_LOG_H - 1)];
   __shared__ Carry128 gs[(1 << SMALL_LOG_H) / SEG_ELEMS];
   const TestDesc td = tests[blockIdx.x];
-  e61* __restrict__ z61 = td.z61;
+  M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   const DigitGen D = td.dgen;
   const int logH = ps.A;
   const int h = 1 << logH;
@@ -476,7 +515,7 @@ __global__ void __launch_bounds__(256) k_small(const TestDesc* __restrict__ test
   const uint32_t j0[1] = {0};
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   stage_range(S61, S31, sq61, sq31, 0, h >> 1);
-  for (int i = threadIdx.x; i < h; i += blockDim.x) L61[PADX(i)] = z61[i];
+  for (int i = threadIdx.x; i < h; i += blockDim.x) L61[PADX(i)] = gld(z61 + i);
   __syncthreads();
   for (uint32_t it = 0; it < td.iters; ++it) {
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_weight(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D);
@@ -493,7 +532,7 @@ __global__ void __launch_bounds__(256) k_small(const TestDesc* __restrict__ test
       seg_ripple(L61, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D, load_carry(gs, (s + nseg - 1) % nseg));
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < h; i += blockDim.x) z61[i] = L61[PADX(i)];
+  for (int i = threadIdx.x; i < h; i += blockDim.x) gst(z61 + i, L61[PADX(i)]);
 }
 
 // ---------------------------------------------------------------------------
@@ -534,10 +573,10 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   __shared__ e31 T31[OUTER_TW];
   __shared__ Carry128 gcin[PASS_MAX_ELEMS / SEG_ELEMS];
   const TestDesc td = tests[blockIdx.y];
-  const e61* __restrict__ zd = td.zd;
-  e61* __restrict__ z61 = td.z61;
-  e31* __restrict__ z31 = td.z31;
-  const Carry128* __restrict__ G = td.G;
+  const M2_GLOBAL e61* __restrict__ zd = gptr(td.zd);
+  M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
+  M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
+  const M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
   const DigitGen D = td.dgen;
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
@@ -545,9 +584,9 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   const uint32_t j0[1] = {0};
   // digits of the strip are contiguous (loads batched), then the carry
   // entering each strip row
-  const e61* src = zd + (size_t)cg * (R * W);
+  const M2_GLOBAL e61* src = zd + (size_t)cg * (R * W);
   gather1<4>(L61, src, R * W, [](int i) { return (uint32_t)i; }, [](int i) { return PADX(i); });
-  for (int rr = threadIdx.x; rr < R; rr += blockDim.x) gcin[rr] = G[prev_strip_carry(cg, rr, R, NS)];
+  for (int rr = threadIdx.x; rr < R; rr += blockDim.x) gcin[rr] = gld(G + prev_strip_carry(cg, rr, R, NS));
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   // prologue: carry-in of each strip-row, then weight
@@ -563,8 +602,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   lds_pass<true, true>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
-    z61[e] = L61[PADX(i)];
-    z31[e] = L31[PADX(i)];
+    gst(z61 + e, L61[PADX(i)]);
+    gst(z31 + e, L31[PADX(i)]);
   }
 }
 
@@ -576,10 +615,10 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   __shared__ e31 T31[OUTER_TW];
   __shared__ Carry128 gs[PASS_MAX_ELEMS / SEG_ELEMS];
   const TestDesc td = tests[blockIdx.y];
-  const e61* __restrict__ z61 = td.z61;
-  const e31* __restrict__ z31 = td.z31;
-  e61* __restrict__ zd = td.zd;
-  Carry128* __restrict__ G = td.G;
+  const M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
+  const M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
+  M2_GLOBAL e61* __restrict__ zd = gptr(td.zd);
+  M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
   const DigitGen D = td.dgen;
   const int mode = td.mode;
   const int R = 1 << ps.A, W = 1 << logW;
@@ -606,11 +645,11 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
     const int idx0 = r * W + v * SEG_ELEMS;
     const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
     if (v > 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gs, it - 1));
-    if (v == subs - 1) G[cg * (uint32_t)R + (uint32_t)r] = gs[it];
+    if (v == subs - 1) gst(G + cg * (uint32_t)R + (uint32_t)r, gs[it]);
   }
   __syncthreads();
-  e61* dst = zd + (size_t)cg * (R * W);
-  for (int i = threadIdx.x; i < R * W; i += blockDim.x) dst[i] = L61[PADX(i)];
+  M2_GLOBAL e61* dst = zd + (size_t)cg * (R * W);
+  for (int i = threadIdx.x; i < R * W; i += blockDim.x) gst(dst + i, L61[PADX(i)]);
 }
 
 // Middle pass: levels [ell0, ell0 + A) with no carry work. Level-ell0 block
@@ -623,8 +662,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
   const TestDesc td = tests[blockIdx.y];
-  e61* __restrict__ z61 = td.z61;
-  e31* __restrict__ z31 = td.z31;
+  M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
+  M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW;
   const uint32_t bx = xcd_strip(blockIdx.x, gridDim.x);
@@ -639,8 +678,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
   lds_pass<FWD, true>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1));
-    z61[e] = L61[PADX(i)];
-    z31[e] = L31[PADX(i)];
+    gst(z61 + e, L61[PADX(i)]);
+    gst(z31 + e, L31[PADX(i)]);
   }
 }
 
@@ -657,8 +696,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
   __shared__ e61 S61[1 << CENTRE_LOG_C];
   __shared__ e31 S31[1 << CENTRE_LOG_C];
   const TestDesc td = tests[blockIdx.y];
-  e61* __restrict__ z61 = td.z61;
-  e31* __restrict__ z31 = td.z31;
+  M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
+  M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
   const int logC = ps.A, C = 1 << logC;
   const uint32_t g = blockIdx.x;
   uint32_t B = 0, B2 = 1;
@@ -713,8 +752,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
   lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31);
   for (int i = threadIdx.x; i < (nb << logC); i += blockDim.x) {
     const uint32_t e = ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1));
-    z61[e] = L61[PADX(i)];
-    z31[e] = L31[PADX(i)];
+    gst(z61 + e, L61[PADX(i)]);
+    gst(z31 + e, L31[PADX(i)]);
   }
 }
 
