@@ -5,6 +5,8 @@ whole residue, p drawn from seed 1 in [18874379, 36700159] (p = 27899687).
 The run first checks Res64 at iterations 500/1000/2000 against the exact
 fixture (tests/golden/c2.json), then times `--steps` more iterations after
 `--warmup` untimed ones, with the residue resident in HBM throughout.
+`--workload ll-n2p24` measures the n = 2^24 Lucas-Lehmer test the same way
+(p from seed 1 in [285212677, 553648103], Res64 checked at 100 and 1000).
 
 Prints one JSON line. Extra fields:
   roofline      dominant kernel's algorithmic HBM bytes per launch / its mean
@@ -54,7 +56,7 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(p: int, fixture_case, iters: int):
+def cpu_baseline(p: int, mode: str, fixture_case, iters: int):
     """Times the oracle (CPU restatement, 1 thread) for `iters` iterations and
     checks its Res64 against the fixture when the fixture has that checkpoint."""
     so = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -64,7 +66,7 @@ def cpu_baseline(p: int, fixture_case, iters: int):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_ctypes import Oracle
     o = Oracle(so)
-    t = o.L.m2o_test_new(p, 1, 64)
+    t = o.L.m2o_test_new(p, 1 if mode == "prp" else 0, 64)
     try:
         o.L.m2o_test_run(t, 1)  # first iteration outside the sample (cache warm-up)
         t0 = time.perf_counter()
@@ -78,22 +80,40 @@ def cpu_baseline(p: int, fixture_case, iters: int):
         ok = fixture_case["checkpoints"][str(iters + 1)]["res64"] == res
     return {"value": dt / iters * 1e6, "unit": "µs/iter", "cores": 1, "kind": "port",
             "sample": "oracle/m2_oracle.c (CPU restatement of the reference's radix-4 IBDWT, 1 thread), "
-                      "iterations 2..%d of the same PRP test on this host; Res64 at %d %s" % (
-                          iters + 1, iters + 1, {True: "matches", False: "MISMATCH", None: "unchecked"}[ok])}
+                      "iterations 2..%d of the same %s test on this host; Res64 at %d %s" % (
+                          iters + 1, mode.upper(), iters + 1,
+                          {True: "matches", False: "MISMATCH", None: "unchecked"}[ok])}
+
+
+# Workloads: exponent range, mode, exact fixture and its checked iterations.
+# The default (the bench line) is the plain PRP test at n = 2^20; the n = 2^24
+# LL test is measured the same way on request.
+WORKLOADS = {
+    "prp-n2p20": {"range": "C2_RANGE", "mode": "prp", "fixture": "c2.json", "checks": (500, 1000, 2000),
+                  "steps": 100000, "cpu_iters": 9, "traffic_key": "c2"},
+    "ll-n2p24": {"range": "C3_RANGE", "mode": "ll", "fixture": "c3.json", "checks": (100, 1000),
+                 "steps": 5000, "cpu_iters": 2, "traffic_key": "c3"},
+}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="prp-n2p20")
+    ap.add_argument("--steps", type=int, default=None, help="timed iterations (default per workload)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--kernel-iters", type=int, default=200, help="iterations timed kernel by kernel")
-    ap.add_argument("--cpu-iters", type=int, default=9, help="oracle iterations in the CPU baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=None, help="oracle iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device 3 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    wl = WORKLOADS[args.workload]
+    if args.steps is None:
+        args.steps = wl["steps"]
+    if args.cpu_iters is None:
+        args.cpu_iters = wl["cpu_iters"]
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -114,26 +134,27 @@ def main() -> int:
     import mersenne2gpu_amd as m2
     from mersenne2gpu_amd import workloads as W
 
-    p = W.draw_uniform(args.seed, *W.C2_RANGE)
+    mode = wl["mode"]
+    p = W.draw_uniform(args.seed, *getattr(W, wl["range"]))
     ln = W.transformsize(p)
     n, h = 1 << ln, 1 << (ln - 1)
     fixture = None
-    fx_path = os.path.join(ROOT, "tests", "golden", "c2.json")
+    fx_path = os.path.join(ROOT, "tests", "golden", wl["fixture"])
     if os.path.exists(fx_path):
         with open(fx_path) as f:
             for c in json.load(f)["cases"]:
-                if c["p"] == p and c["mode"] == "prp":
+                if c["p"] == p and c["mode"] == mode:
                     fixture = c
 
     ctx = m2.Context(0)
     log("device:", ctx.name, ctx.props())
-    t = m2.Test(ctx, p, "prp")
+    t = m2.Test(ctx, p, mode)
     log(t.plan().rstrip())
 
-    # parity gate at 500/1000/2000
+    # parity gate at the fixture's checkpoints
     parity = {}
     done = 0
-    for k in (500, 1000, 2000):
+    for k in wl["checks"]:
         t.run(k - done)
         done = k
         got = t.res64()
@@ -158,14 +179,14 @@ def main() -> int:
     dom = max(per_kernel_us, key=per_kernel_us.get)
     bytes_dom = algorithmic_bytes(dom, h)
     achieved = bytes_dom / (per_kernel_us[dom] * 1e-6) / 1e9
-    traffic = load_traffic(dom, "c2")
+    traffic = load_traffic(dom, wl["traffic_key"])
     whole_gbps = 48.0 * n / (us_per_iter * 1e-6) / 1e9
     t.close()
     ctx.close()
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(p, fixture, args.cpu_iters)
+        cpu = cpu_baseline(p, mode, fixture, args.cpu_iters)
         log("cpu baseline:", cpu)
 
     out = {
@@ -180,11 +201,12 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
-        "data": "synthetic (PRP base 3 from the residue 3; the only inputs are p and the mode)",
-        "config": {"workload": "plain single PRP test, p=%d (seed %d), n=2^%d, GF(M61^2) x GF(M31^2), "
-                               "%d iterations timed after 2000 checked + %d warm-up" % (
-                                   p, args.seed, ln, args.steps, args.warmup),
-                   "p": p, "n": n, "mode": "prp", "seed": args.seed, "parallelism": "1 GPU, 1 test"},
+        "data": "synthetic (%s; the only inputs are p and the mode)" % (
+            "PRP base 3 from the residue 3" if mode == "prp" else "Lucas-Lehmer from 4"),
+        "config": {"workload": "plain single %s test, p=%d (seed %d), n=2^%d, GF(M61^2) x GF(M31^2), "
+                               "%d iterations timed after %d checked + %d warm-up" % (
+                                   mode.upper(), p, args.seed, ln, args.steps, wl["checks"][-1], args.warmup),
+                   "p": p, "n": n, "mode": mode, "seed": args.seed, "parallelism": "1 GPU, 1 test"},
         "iters_per_s": round(1e6 / us_per_iter, 1),
         "hbm_gbps_algorithmic": round(whole_gbps, 1),
         "hbm_frac_algorithmic": round(whole_gbps / HBM_PEAK_GBPS, 4),

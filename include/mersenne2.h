@@ -150,12 +150,17 @@ int m2_batch_residue(m2_batch *b, size_t i, uint8_t *buf, size_t len);
 int m2_batch_is_prime(m2_batch *b, size_t i, int *verdict);
 
 /*
- * Server: a request queue over resident tests. A request is admitted, in
- * arrival order, once its device footprint (m2_test_footprint) fits in the
- * budget; each m2_server_step gives every admitted test one time slice of
- * `slice_iters` iterations, then retires finished tests (their Res64 and
- * verdict go to the completion queue read by m2_server_poll) and frees their
- * memory. No counterpart in the reference, which tests one exponent per run.
+ * Server: a request queue over resident tests (continuous batching). A
+ * request is admitted, in arrival order, once its device footprint
+ * (m2_test_footprint) fits in the budget. Admitted tests of one transform
+ * length form a lane that runs on its own stream, one slice at a time: a
+ * slice advances every member of the lane by at most `slice_iters`
+ * iterations and by at most about 4 ms of device time (measured per lane),
+ * so long tests never hold the device while short ones wait. When a lane's
+ * slice ends, m2_server_step retires its finished tests (their Res64 and
+ * verdict go to the completion queue read by m2_server_poll; their memory is
+ * freed) and starts the lane's next slice with the tests admitted meanwhile.
+ * No counterpart in the reference, which tests one exponent per run.
  * m2_server_submit / _poll / _stats may be called from other threads while
  * one thread drives m2_server_step.
  */
@@ -167,7 +172,8 @@ int m2_server_destroy(m2_server *s);
 /* Queue a test of 2^p - 1; iters 0 means the full test, else the test stops
  * after `iters` iterations (its verdict is then that of the partial state). */
 int m2_server_submit(m2_server *s, uint64_t p, int mode, uint64_t iters, uint64_t *id);
-/* Admit, run one slice, retire. *busy = 1 while requests remain queued or active. */
+/* Admit, retire and start slices on lanes whose slice has ended (waits up to
+ * about 1 ms for one to end). *busy = 1 while requests remain queued or active. */
 int m2_server_step(m2_server *s, int *busy);
 /* Take up to `max` completions (any of ids/res64/verdicts may be NULL). */
 int m2_server_poll(m2_server *s, uint64_t *ids, uint64_t *res64, int *verdicts, size_t max, size_t *count);

@@ -1056,8 +1056,10 @@ struct m2_server {
   struct Lane {
     Group g;
     std::vector<Member> members;
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev0 = nullptr, ev = nullptr;  // around the slice in flight
     bool inflight = false;
+    uint64_t k = 0;          // iterations per member in the slice in flight
+    double us_per_iter = 0;  // measured device time of one iteration of all members
   };
   std::map<Class*, Lane> lanes;
   uint64_t nactive = 0;
@@ -1096,7 +1098,8 @@ int server_admit(m2_server* s, bool* progressed) {
       m2_server::Lane lane;
       if ((rc = lane.g.create(cls))) return rc;
       if (o.graph_iters) lane.g.graph_iters = o.graph_iters;
-      HIPCHK(hipEventCreateWithFlags(&lane.ev, hipEventDisableTiming));
+      HIPCHK(hipEventCreate(&lane.ev0));
+      HIPCHK(hipEventCreate(&lane.ev));
       it = s->lanes.emplace(cls, lane).first;
     }
     TestObj* t = nullptr;
@@ -1112,6 +1115,11 @@ int server_admit(m2_server* s, bool* progressed) {
 }
 
 // Lane whose slice is done (or idle): retire finished members, start the next slice.
+// Device time a lane's slice may take: a slice is at most `slice` iterations
+// and at most about this long, so that no lane holds a hardware queue for
+// long while others (short tests) wait behind it.
+constexpr double SLICE_TARGET_US = 4000.0;
+
 int server_service_lane(m2_server* s, m2_server::Lane& lane, bool* progressed) {
   if (lane.inflight) {
     hipError_t q = hipEventQuery(lane.ev);
@@ -1119,6 +1127,11 @@ int server_service_lane(m2_server* s, m2_server::Lane& lane, bool* progressed) {
     if (q != hipSuccess) return fail(M2_E_HIP, std::string("hipEventQuery: ") + hipGetErrorString(q));
     lane.inflight = false;
     *progressed = true;
+    float ms = 0;
+    if (lane.k && hipEventElapsedTime(&ms, lane.ev0, lane.ev) == hipSuccess && ms > 0) {
+      const double u = ms * 1e3 / (double)lane.k;
+      lane.us_per_iter = lane.us_per_iter > 0 ? 0.5 * (lane.us_per_iter + u) : u;
+    }
   }
   int rc = lane.g.sync();
   if (rc) return rc;
@@ -1144,7 +1157,13 @@ int server_service_lane(m2_server* s, m2_server::Lane& lane, bool* progressed) {
   if (lane.members.empty()) return M2_OK;
   std::vector<TestObj*> ts;
   for (auto& m : lane.members) ts.push_back(m.t);
-  if ((rc = lane.g.advance(ts, s->slice, 0))) return rc;
+  // first slice of a lane: a short probe; then sized by the measured time
+  uint64_t k = s->slice;
+  if (lane.us_per_iter <= 0) k = std::min<uint64_t>(k, 16);
+  else k = std::max<uint64_t>(1, std::min<uint64_t>(k, (uint64_t)(SLICE_TARGET_US / lane.us_per_iter)));
+  lane.k = k;
+  HIPCHK(hipEventRecord(lane.ev0, lane.g.stream));
+  if ((rc = lane.g.advance(ts, k, 0))) return rc;
   HIPCHK(hipEventRecord(lane.ev, lane.g.stream));
   lane.inflight = true;
   return M2_OK;
@@ -1175,6 +1194,7 @@ int m2_server_destroy(m2_server* s) {
     kv.second.g.destroy();
     for (auto& m : kv.second.members) test_obj_destroy(m.t);
     if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+    if (kv.second.ev0) (void)hipEventDestroy(kv.second.ev0);
   }
   if (s->setup) (void)hipStreamDestroy(s->setup);
   delete s;
