@@ -114,6 +114,16 @@ int m2_test_kernel_times(m2_test *t, uint64_t iters, double *ms, int max_kernels
 /* Human-readable plan (the reference's --debug-plan, mersenne2_gpu.cpp:4767-4776). */
 int m2_test_plan(m2_test *t, char *buf, size_t len);
 
+/* Checkpoints: the canonical residue with p, mode, iteration and Res64 in a
+ * small checksummed file (absent in the reference, which cannot stop and
+ * resume a test). Save writes a temporary file and renames it over `path`.
+ * Load needs a test of the same p and mode and continues from the saved
+ * iteration; a damaged or foreign file is refused (M2_E_INVAL). Info reads a
+ * file's header on the host, without a device. */
+int m2_checkpoint_save(m2_test *t, const char *path);
+int m2_checkpoint_load(m2_test *t, const char *path);
+int m2_checkpoint_info(const char *path, uint64_t *p, int *mode, uint64_t *iteration, uint64_t *res64);
+
 /* Autotuner: for the transform length of p and a batch of `ntests` tests,
  * time the candidate plans (single workgroup or multi-pass; centre size, carry
  * block width, radix, then graph length) and write the fastest into *out

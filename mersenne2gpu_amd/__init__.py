@@ -48,6 +48,10 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "m2_test_kernel_times": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double), _c.c_int, _c.POINTER(_c.c_int),
                                         _c.c_char_p, _c.c_size_t]),
     "m2_test_plan": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
+    "m2_checkpoint_save": (_c.c_int, [_P, _c.c_char_p]),
+    "m2_checkpoint_load": (_c.c_int, [_P, _c.c_char_p]),
+    "m2_checkpoint_info": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
+                                      _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64)]),
     "m2_autotune": (_c.c_int, [_P, _c.c_uint64, _c.c_uint32, _P, _c.c_char_p, _c.c_size_t]),
     "m2_batch_create": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
                                    _c.POINTER(_c.c_uint64), _P, _c.POINTER(_P)]),
@@ -152,6 +156,15 @@ class Context:
         return Test(self, p, mode, graph_iters)
 
 
+def checkpoint_info(path: str) -> Dict[str, object]:
+    """Header of a checkpoint file (m2_checkpoint_info; no device needed)."""
+    p, mode, it, r = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_uint64(), ctypes.c_uint64()
+    _chk(lib().m2_checkpoint_info(os.fsencode(path), ctypes.byref(p), ctypes.byref(mode), ctypes.byref(it),
+                                  ctypes.byref(r)), "m2_checkpoint_info")
+    return {"p": p.value, "mode": "prp" if mode.value == MODE_PRP else "ll", "iteration": it.value,
+            "res64": "%016X" % r.value}
+
+
 def autotune(ctx: "Context", p: int, ntests: int = 1) -> Tuple[Dict[str, int], str]:
     """Plan options the autotuner picks for p's transform length and a batch of
     ntests tests, and its per-candidate report (m2_autotune)."""
@@ -242,6 +255,12 @@ class Test:
         v = ctypes.c_int()
         _chk(lib().m2_test_is_prime(self.h, ctypes.byref(v)), "m2_test_is_prime")
         return bool(v.value)
+
+    def save_checkpoint(self, path: str) -> None:
+        _chk(lib().m2_checkpoint_save(self.h, os.fsencode(path)), "m2_checkpoint_save")
+
+    def load_checkpoint(self, path: str) -> None:
+        _chk(lib().m2_checkpoint_load(self.h, os.fsencode(path)), "m2_checkpoint_load")
 
     def time(self, iters: int) -> float:
         """Device milliseconds for `iters` iterations (hipEvents)."""

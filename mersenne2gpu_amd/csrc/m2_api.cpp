@@ -685,6 +685,119 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
   return M2_OK;
 }
 
+// ---- checkpoints ----
+// File layout (little-endian): magic "M2CKPT01", u32 p, u32 mode, u64
+// iteration, u64 Res64, u64 n = ceil(p/8), n residue bytes (canonical R),
+// u64 FNV-1a hash of every preceding byte.
+
+static uint64_t fnv1a(const uint8_t* d, size_t n) {
+  uint64_t h = 1469598103934665603ull;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= d[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+static void put_u32(std::vector<uint8_t>& b, uint32_t v) {
+  for (int i = 0; i < 4; ++i) b.push_back((uint8_t)(v >> (8 * i)));
+}
+static void put_u64(std::vector<uint8_t>& b, uint64_t v) {
+  for (int i = 0; i < 8; ++i) b.push_back((uint8_t)(v >> (8 * i)));
+}
+static uint64_t get_u64(const uint8_t* d) {
+  uint64_t v = 0;
+  for (int i = 7; i >= 0; --i) v = (v << 8) | d[i];
+  return v;
+}
+static uint32_t get_u32(const uint8_t* d) {
+  uint32_t v = 0;
+  for (int i = 3; i >= 0; --i) v = (v << 8) | d[i];
+  return v;
+}
+
+struct CkptData {
+  uint32_t p = 0;
+  int mode = 0;
+  uint64_t iter = 0, res64 = 0;
+  std::vector<uint8_t> residue;
+};
+
+static int ckpt_read(const char* path, CkptData& c) {
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(M2_E_INVAL, std::string("cannot open checkpoint ") + path);
+  std::vector<uint8_t> b;
+  uint8_t tmp[65536];
+  size_t k;
+  while ((k = fread(tmp, 1, sizeof tmp, f)) > 0) b.insert(b.end(), tmp, tmp + k);
+  fclose(f);
+  const size_t head = 8 + 4 + 4 + 8 + 8 + 8;
+  if (b.size() < head + 8 || memcmp(b.data(), "M2CKPT01", 8) != 0)
+    return fail(M2_E_INVAL, std::string("not a checkpoint file: ") + path);
+  c.p = get_u32(&b[8]);
+  c.mode = (int)get_u32(&b[12]);
+  c.iter = get_u64(&b[16]);
+  c.res64 = get_u64(&b[24]);
+  const uint64_t n = get_u64(&b[32]);
+  if (c.p < 3 || (c.mode != M2_MODE_LL && c.mode != M2_MODE_PRP) || n != (c.p + 7u) / 8u ||
+      b.size() != head + n + 8)
+    return fail(M2_E_INVAL, std::string("malformed checkpoint: ") + path);
+  if (fnv1a(b.data(), head + n) != get_u64(&b[head + n]))
+    return fail(M2_E_INVAL, std::string("checkpoint checksum mismatch: ") + path);
+  c.residue.assign(b.begin() + head, b.begin() + head + n);
+  uint64_t r = 0;
+  for (int i = 7; i >= 0; --i) r = (r << 8) | (i < (int)n ? c.residue[i] : 0);
+  if (r != c.res64) return fail(M2_E_INVAL, std::string("checkpoint Res64 does not match its residue: ") + path);
+  return M2_OK;
+}
+
+int m2_checkpoint_save(m2_test* t, const char* path) {
+  if (!t || !path) return fail(M2_E_INVAL, "null pointer");
+  std::vector<uint8_t> r;
+  int rc = t->group.sync();
+  if (rc) return rc;
+  if ((rc = test_obj_residue(t->obj, r))) return rc;
+  std::vector<uint8_t> b;
+  b.insert(b.end(), "M2CKPT01", "M2CKPT01" + 8);
+  put_u32(b, t->obj->p);
+  put_u32(b, (uint32_t)t->obj->mode);
+  put_u64(b, t->obj->iter);
+  put_u64(b, res64_of(r));
+  put_u64(b, r.size());
+  b.insert(b.end(), r.begin(), r.end());
+  put_u64(b, fnv1a(b.data(), b.size()));
+  // write-then-rename, so an interrupted save leaves the previous checkpoint
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return fail(M2_E_INVAL, "cannot write " + tmp);
+  const bool ok = fwrite(b.data(), 1, b.size(), f) == b.size();
+  if (fclose(f) != 0 || !ok) return fail(M2_E_INVAL, "cannot write " + tmp);
+  if (rename(tmp.c_str(), path) != 0) return fail(M2_E_INVAL, std::string("cannot rename to ") + path);
+  return M2_OK;
+}
+
+int m2_checkpoint_load(m2_test* t, const char* path) {
+  if (!t || !path) return fail(M2_E_INVAL, "null pointer");
+  CkptData c;
+  int rc = ckpt_read(path, c);
+  if (rc) return rc;
+  if (c.p != t->obj->p || c.mode != t->obj->mode)
+    return fail(M2_E_INVAL, "checkpoint is for p=" + std::to_string(c.p) + (c.mode ? " prp" : " ll") +
+                                ", not this test");
+  return m2_test_set_residue(t, c.residue.data(), c.residue.size(), c.iter);
+}
+
+int m2_checkpoint_info(const char* path, uint64_t* p, int* mode, uint64_t* iteration, uint64_t* res64) {
+  if (!path) return fail(M2_E_INVAL, "null pointer");
+  CkptData c;
+  int rc = ckpt_read(path, c);
+  if (rc) return rc;
+  if (p) *p = c.p;
+  if (mode) *mode = c.mode;
+  if (iteration) *iteration = c.iter;
+  if (res64) *res64 = c.res64;
+  return M2_OK;
+}
+
 int m2_test_plan(m2_test* t, char* buf, size_t len) {
   if (!t || !buf || !len) return fail(M2_E_INVAL, "null pointer");
   snprintf(buf, len, "%s", describe_plan(t->obj->cls->P).c_str());

@@ -1,7 +1,7 @@
 // mersenne2 <p> [ll|prp] [report_every] [-d id] [-R sec] [-W wg] [-CW wg] [-B carry_pairs]
 //           [-LS cap] [-N max_iters] [--show-defaults] [--enable-chunk64] [--debug-plan]
 //           [--debug-stage-selftest] [--debug-cppint-every N] [--debug-cppint-max-iter N]
-//           [--res64-at k1,k2,...] [--dump-residue path]
+//           [--res64-at k1,k2,...] [--dump-residue path] [--checkpoint path] [--resume path]
 //
 // Drop-in for the reference's command line (mersenne2_gpu.cpp:3618-4960): same
 // grammar, defaults, stdout lines and exit codes, on top of the libm2 C ABI. It
@@ -30,7 +30,7 @@ const char* USAGE_TAIL =
     " <p> [ll|prp] [report_every] [-d device_id] [-R report_seconds] [-W fft_wg] [-CW carry_wg] [-B carry_pairs] "
     "[-LS local_stage_cap] [-N max_iters] [--show-defaults] [--enable-chunk64] [--debug-plan] "
     "[--debug-stage-selftest] [--debug-cppint-every N] [--debug-cppint-max-iter N] [--res64-at k1,k2,...] "
-    "[--dump-residue path]\n";
+    "[--dump-residue path] [--checkpoint path] [--resume path]\n";
 
 int die(const char* what, int code) {
   std::cerr << what << " failed with error " << code << ": " << m2_last_error() << std::endl;
@@ -60,7 +60,7 @@ int main(int argc, char* argv[]) {
   bool show_defaults = false, debug_plan = false, debug_stage_selftest = false, enable_chunk64 = false;
   uint32_t debug_cppint_every = 0, debug_cppint_max_iter = 0;
   std::vector<uint64_t> res64_at;
-  std::string dump_path;
+  std::string dump_path, checkpoint_path, resume_path;
 
   auto need = [&](int i, const char* msg) -> bool {
     if (i + 1 >= argc) {
@@ -119,6 +119,12 @@ int main(int argc, char* argv[]) {
     } else if (arg == "--dump-residue") {
       if (!need(i, "Missing path after --dump-residue")) return 1;
       dump_path = argv[++i];
+    } else if (arg == "--checkpoint") {
+      if (!need(i, "Missing path after --checkpoint")) return 1;
+      checkpoint_path = argv[++i];
+    } else if (arg == "--resume") {
+      if (!need(i, "Missing path after --resume")) return 1;
+      resume_path = argv[++i];
     } else if (!report_every_set) {
       report_every = (uint32_t)std::strtoul(arg.c_str(), nullptr, 10);
       report_every_set = true;
@@ -212,23 +218,31 @@ int main(int argc, char* argv[]) {
   if (max_iters_override != 0u) total = std::min<uint64_t>(total, max_iters_override);
   if (max_iters_override != 0u) std::cout << "benchmark_iters=" << total << "\n";
 
+  uint64_t start = 0;
+  if (!resume_path.empty()) {
+    rc = m2_checkpoint_load(t, resume_path.c_str());
+    if (rc != M2_OK) return die("m2_checkpoint_load", rc);
+    m2_test_iteration(t, &start);
+    if (start > total) start = total;
+    std::cout << "resumed from " << resume_path << " at iteration " << start << "\n";
+  }
   const auto t0 = std::chrono::steady_clock::now();
   auto report = [&](uint64_t iter) {
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     const double pct = total ? 100.0 * (double)iter / (double)total : 100.0;
-    const double itps = el > 0 ? (double)iter / el : 0.0;
+    const double itps = el > 0 ? (double)(iter - start) / el : 0.0;
     const double eta = itps > 0 ? (double)(total - iter) / itps : std::numeric_limits<double>::infinity();
     std::cout << "iter " << iter << "/" << total << " (" << std::fixed << std::setprecision(1) << pct << "%)"
               << ", elapsed " << std::setprecision(2) << el << " s, it/s " << std::setprecision(1) << itps;
     if (std::isfinite(eta)) std::cout << ", ETA " << std::setprecision(1) << eta << " s";
     std::cout << "\n" << std::flush;
   };
-  report(0);
+  report(start);
   const uint64_t batch = std::max<uint32_t>(1u, report_every ? report_every : 1000u);
-  uint64_t iter = 0;
+  uint64_t iter = start;
   size_t next_cp = 0;
   while (iter < total) {
-    uint64_t chunk_end = std::min<uint64_t>(total, iter + batch);
+    uint64_t chunk_end = std::min<uint64_t>(total, (iter / batch + 1) * batch);
     if (next_cp < res64_at.size() && res64_at[next_cp] > iter && res64_at[next_cp] < chunk_end)
       chunk_end = res64_at[next_cp];
     rc = m2_test_run(t, chunk_end - iter);
@@ -243,7 +257,13 @@ int main(int argc, char* argv[]) {
       }
       ++next_cp;
     }
-    if (iter % batch == 0 || iter == total) report(iter);
+    if (iter % batch == 0 || iter == total) {
+      report(iter);
+      if (!checkpoint_path.empty()) {
+        rc = m2_checkpoint_save(t, checkpoint_path.c_str());
+        if (rc != M2_OK) return die("m2_checkpoint_save", rc);
+      }
+    }
   }
   rc = m2_test_sync(t);
   if (rc != M2_OK) return die("m2_test_sync", rc);
@@ -252,7 +272,7 @@ int main(int argc, char* argv[]) {
   uint64_t r64 = 0;
   if (m2_test_res64(t, &r64) != M2_OK) return die("m2_test_res64", -1);
   const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  const double itps = el > 0 ? (double)total / el : 0.0;
+  const double itps = el > 0 ? (double)(total - start) / el : 0.0;
   std::cout << "2^" << p << " - 1 is "
             << (prime ? (mode == M2_MODE_LL ? "prime" : "a probable prime") : "composite") << ", elapsed "
             << std::fixed << std::setprecision(2) << el << " s, it/s " << std::setprecision(1) << itps << "\n";
