@@ -40,7 +40,8 @@ def algorithmic_bytes(kernel: str, h: int) -> int:
     """HBM bytes a kernel must move per launch: the 24-byte elements it reads
     and writes once (digits are 16 bytes per element)."""
     per_elem = {"k_centre": 2 * ELEM_BYTES, "k_mid_fwd": 2 * ELEM_BYTES, "k_mid_inv": 2 * ELEM_BYTES,
-                "k_outer_fwd": 16 + ELEM_BYTES, "k_outer_inv": ELEM_BYTES + 16, "k_small": 0}
+                "k_outer_fwd": 16 + ELEM_BYTES, "k_outer_inv": ELEM_BYTES + 16, "k_outer_fused": 2 * ELEM_BYTES,
+                "k_small": 0}
     return per_elem.get(kernel, 2 * ELEM_BYTES) * h
 
 
@@ -174,7 +175,7 @@ def main() -> int:
 
     # per-kernel device time (each kernel's dispatch start/stop timestamps)
     kt = t.kernel_times(args.kernel_iters)
-    per_kernel_us = {k: v * 1000.0 / args.kernel_iters for k, v in kt.items()}
+    per_kernel_us = {k: v * 1000.0 for k, v in kt.items()}  # mean per launch
     log("per-kernel us:", json.dumps({k: round(v, 3) for k, v in per_kernel_us.items()}))
     dom = max(per_kernel_us, key=per_kernel_us.get)
     bytes_dom = algorithmic_bytes(dom, h)

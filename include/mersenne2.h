@@ -53,7 +53,8 @@ typedef struct m2_opts {
   uint32_t max_radix;       /* 8 (0: default) or 4: largest radix of the transform steps */
   uint32_t autotune;        /* batches and servers: choose the four fields above per transform
                                length by timing candidates (m2_autotune) */
-  uint32_t reserved;
+  uint32_t no_fuse;         /* single tests: 1 = keep the outer passes of consecutive iterations
+                               in separate launches (0: fuse them where the plan allows) */
 } m2_opts;
 
 /* Message of the calling thread's last failure ("" if none). */
@@ -105,10 +106,9 @@ int m2_test_is_prime(m2_test *t, int *verdict);
 /* Run `iters` iterations and return their wall time on the device in ms
  * (hipEvents around the launches). */
 int m2_test_time(m2_test *t, uint64_t iters, double *ms);
-/* Run `iters` iterations kernel by kernel, recording each kernel's dispatch
- * start/stop timestamps, and accumulate per-kernel device time. names_buf
- * receives the kernel names separated by '\n'; ms[i] the total milliseconds
- * of kernel i. */
+/* Run `iters` iterations, recording every launch's dispatch start/stop
+ * timestamps. names_buf receives the kernel names separated by '\n'; ms[i]
+ * is kernel i's mean device time per launch, in milliseconds. */
 int m2_test_kernel_times(m2_test *t, uint64_t iters, double *ms, int max_kernels, int *count, char *names_buf,
                          size_t names_len);
 /* Human-readable plan (the reference's --debug-plan, mersenne2_gpu.cpp:4767-4776). */

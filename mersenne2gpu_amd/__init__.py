@@ -177,10 +177,10 @@ def autotune(ctx: "Context", p: int, ntests: int = 1) -> Tuple[Dict[str, int], s
 class _Opts(ctypes.Structure):
     _fields_ = [("wg", ctypes.c_uint32), ("carry_wg", ctypes.c_uint32), ("carry_pairs", ctypes.c_uint32),
                 ("local_stage_cap", ctypes.c_uint32), ("graph_iters", ctypes.c_uint32),
-                ("max_radix", ctypes.c_uint32), ("autotune", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("max_radix", ctypes.c_uint32), ("autotune", ctypes.c_uint32), ("no_fuse", ctypes.c_uint32)]
 
     def as_dict(self) -> Dict[str, int]:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def make_opts(**kw) -> "_Opts":
@@ -269,17 +269,14 @@ class Test:
         return v.value
 
     def kernel_times(self, iters: int) -> Dict[str, float]:
-        """Total device milliseconds per kernel over `iters` iterations."""
-        ms = (ctypes.c_double * 8)()
+        """Mean device milliseconds per launch of each kernel over `iters` iterations."""
+        ms = (ctypes.c_double * 16)()
         cnt = ctypes.c_int()
-        names = ctypes.create_string_buffer(512)
-        _chk(lib().m2_test_kernel_times(self.h, int(iters), ms, 8, ctypes.byref(cnt), names, 512),
+        names = ctypes.create_string_buffer(1024)
+        _chk(lib().m2_test_kernel_times(self.h, int(iters), ms, 16, ctypes.byref(cnt), names, 1024),
              "m2_test_kernel_times")
         keys = names.value.decode().split("\n")[: cnt.value]
-        out: Dict[str, float] = {}
-        for k, v in zip(keys, ms[: cnt.value]):
-            out[k] = out.get(k, 0.0) + v
-        return out
+        return {k: v for k, v in zip(keys, ms[: cnt.value])}
 
     def plan(self) -> str:
         buf = ctypes.create_string_buffer(4096)

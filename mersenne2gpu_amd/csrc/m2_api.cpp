@@ -58,6 +58,7 @@ struct Class {
   int ln = 0;
   PlanOpts popts;
   DevPlan P;
+  bool fusable = false;  // single tests may use the fused outer pass (strips fit one workgroup per CU)
   std::vector<void*> allocs;
   uint64_t bytes = 0;
 };
@@ -136,6 +137,7 @@ int get_class(m2_ctx* ctx, int ln, const PlanOpts& po, Class** out) {
   c->P.sq61 = (const e61*)d;
   if ((rc = dev_alloc(c->allocs, c->bytes, &d, T.sq31.data(), T.sq31.size() * sizeof(e31)))) return bail(rc);
   c->P.sq31 = (const e31*)d;
+  c->fusable = !c->P.small && !c->P.has_mid && outer_strips(c->P) <= (uint32_t)ctx->prop.multiProcessorCount;
   if (!c->P.small) {
     if ((rc = dev_alloc(c->allocs, c->bytes, &d, T.ctw61.data(), T.ctw61.size() * sizeof(e61)))) return bail(rc);
     c->P.ctw61 = (const e61*)d;
@@ -199,6 +201,12 @@ int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, const PlanOpts& po, hipSt
     t->desc.zd = (e61*)d;
     if ((rc = dev_alloc(t->allocs, t->bytes, &d, nullptr, P.nseg * sizeof(Carry128)))) return bail(rc);
     t->desc.G = (Carry128*)d;
+  }
+  if (cls->fusable) {
+    const size_t nsync = (outer_strips(P) + 1) * sizeof(uint32_t);
+    if ((rc = dev_alloc(t->allocs, t->bytes, &d, nullptr, nsync))) return bail(rc);
+    t->desc.sync = (uint32_t*)d;
+    HIPCHK(hipMemsetAsync(d, 0, nsync, st));
   }
   t->desc.dgen = t->dgen;
   t->desc.mode = mode;
@@ -266,6 +274,7 @@ struct Group {
   Class* cls = nullptr;
   hipStream_t stream = nullptr;
   uint32_t graph_iters = GRAPH_ITERS;  // used when advance() is given 0
+  bool fused = false;                  // single test: outer passes of consecutive iterations fused
   struct Round {
     TestDesc* d = nullptr;
     uint32_t n = 0;
@@ -342,13 +351,14 @@ struct Group {
     if (rc) return rc;
     if (G > 1) {
       HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      for (uint32_t i = 0; i < G; ++i) {
-        hipError_t e = launch_iteration(cls->P, r.d, r.n, stream);
-        if (e != hipSuccess) {
-          hipGraph_t g;
-          (void)hipStreamEndCapture(stream, &g);
-          return fail(M2_E_HIP, std::string("capture: ") + hipGetErrorString(e));
-        }
+      hipError_t e = hipSuccess;
+      if (use_fused(r.n)) e = launch_run_fused(cls->P, r.d, G, stream);
+      else
+        for (uint32_t i = 0; i < G && e == hipSuccess; ++i) e = launch_iteration(cls->P, r.d, r.n, stream);
+      if (e != hipSuccess) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(stream, &g);
+        return fail(M2_E_HIP, std::string("capture: ") + hipGetErrorString(e));
       }
       HIPCHK(hipStreamEndCapture(stream, &r.graph));
       HIPCHK(hipGraphInstantiate(&r.gexec, r.graph, nullptr, nullptr, 0));
@@ -375,10 +385,13 @@ struct Group {
     uint64_t left = k;
     if (G > 1)
       for (; left >= G; left -= G) HIPCHK(hipGraphLaunch(r->gexec, stream));
-    for (; left; --left) HIPCHK(launch_iteration(cls->P, r->d, r->n, stream));
+    if (left && use_fused(r->n)) HIPCHK(launch_run_fused(cls->P, r->d, left, stream));
+    else
+      for (; left; --left) HIPCHK(launch_iteration(cls->P, r->d, r->n, stream));
     for (TestObj* t : tests) t->iter += k;
     return M2_OK;
   }
+  bool use_fused(uint32_t ntests) const { return fused && ntests == 1 && cls->fusable; }
   // Build what advance() will launch for `tests` (keeps setup out of timings).
   int prepare(const std::vector<TestObj*>& tests, uint32_t graph_iters) {
     if (cls->P.small || tests.size() > MAX_GRID_ROWS) return M2_OK;
@@ -531,7 +544,21 @@ int m2_test_create(m2_ctx* ctx, uint64_t p, int mode, const m2_opts* opts, m2_te
     return rc;
   }
   t->obj->target = ~0ull;  // a single test runs as many iterations as asked
+  t->group.fused = cls->fusable && !(opts && opts->no_fuse);
   *out = t.release();
+  return M2_OK;
+}
+
+// Waits for the test's stream; on the fused plan also checks that no
+// workgroup timed out waiting for a neighbour's carries.
+static int test_sync(m2_test* t) {
+  int rc = t->group.sync();
+  if (rc || !t->obj->desc.sync) return rc;
+  uint32_t fault = 0;
+  HIPCHK(hipMemcpy(&fault, t->obj->desc.sync + outer_strips(t->obj->cls->P), sizeof fault, hipMemcpyDeviceToHost));
+  if (fault)
+    return fail(M2_E_STATE, "fused outer pass: a workgroup timed out waiting for its neighbour's carries "
+                            "(another kernel held the GPU); the test's state is invalid");
   return M2_OK;
 }
 
@@ -553,7 +580,7 @@ int m2_test_run(m2_test* t, uint64_t iters) {
 
 int m2_test_sync(m2_test* t) {
   if (!t) return fail(M2_E_INVAL, "null test");
-  return t->group.sync();
+  return test_sync(t);
 }
 
 int m2_test_iteration(m2_test* t, uint64_t* iter) {
@@ -572,7 +599,7 @@ int m2_test_residue(m2_test* t, uint8_t* buf, size_t len) {
   if (!t || !buf) return fail(M2_E_INVAL, "null pointer");
   const size_t need = (t->obj->p + 7) / 8;
   if (len < need) return fail(M2_E_INVAL, "residue buffer too small: need " + std::to_string(need) + " bytes");
-  int rc = t->group.sync();
+  int rc = test_sync(t);
   if (rc) return rc;
   std::vector<uint8_t> r;
   if ((rc = test_obj_residue(t->obj, r))) return rc;
@@ -583,7 +610,7 @@ int m2_test_residue(m2_test* t, uint8_t* buf, size_t len) {
 
 int m2_test_res64(m2_test* t, uint64_t* res64) {
   if (!t || !res64) return fail(M2_E_INVAL, "null pointer");
-  int rc = t->group.sync();
+  int rc = test_sync(t);
   if (rc) return rc;
   std::vector<uint8_t> r;
   if ((rc = test_obj_residue(t->obj, r))) return rc;
@@ -593,7 +620,7 @@ int m2_test_res64(m2_test* t, uint64_t* res64) {
 
 int m2_test_is_prime(m2_test* t, int* verdict) {
   if (!t || !verdict) return fail(M2_E_INVAL, "null pointer");
-  int rc = t->group.sync();
+  int rc = test_sync(t);
   if (rc) return rc;
   std::vector<uint8_t> r;
   if ((rc = test_obj_residue(t->obj, r))) return rc;
@@ -604,7 +631,7 @@ int m2_test_is_prime(m2_test* t, int* verdict) {
 int m2_test_set_residue(m2_test* t, const uint8_t* buf, size_t len, uint64_t iteration) {
   if (!t || !buf) return fail(M2_E_INVAL, "null pointer");
   HIPCHK(hipSetDevice(t->ctx->device));
-  int rc = t->group.sync();
+  int rc = test_sync(t);
   if (rc) return rc;
   std::vector<int64_t> digits(2ull * t->obj->cls->P.h);
   digits_from_residue(t->obj->p, t->obj->tt.bitpos, t->obj->tt.dig, buf, len, digits.data());
@@ -616,7 +643,7 @@ int m2_test_set_residue(m2_test* t, const uint8_t* buf, size_t len, uint64_t ite
 int m2_test_time(m2_test* t, uint64_t iters, double* ms) {
   if (!t || !ms) return fail(M2_E_INVAL, "null pointer");
   HIPCHK(hipSetDevice(t->ctx->device));
-  int rc = t->group.sync();
+  int rc = test_sync(t);
   if (rc) return rc;
   hipEvent_t a, b;
   if ((rc = t->group.prepare({t->obj}, t->graph_iters))) return rc;
@@ -631,17 +658,20 @@ int m2_test_time(m2_test* t, uint64_t iters, double* ms) {
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   *ms = f;
-  return t->group.sync();
+  return test_sync(t);
 }
 
 int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels, int* count, char* names_buf,
                          size_t names_len) {
   if (!t || !ms || !count) return fail(M2_E_INVAL, "null pointer");
   HIPCHK(hipSetDevice(t->ctx->device));
-  int rc = t->group.sync();
+  int rc = test_sync(t);
   if (rc) return rc;
-  constexpr int MAXE = 8;
-  hipEvent_t starts[MAXE], stops[MAXE];
+  // Runs of up to RUN iterations, every launch between its own dispatch
+  // timestamps; per-kernel totals and launch counts give the mean per launch.
+  constexpr uint64_t RUN = 16;
+  constexpr int MAXE = 2 * (int)RUN + 8;
+  std::vector<hipEvent_t> starts(MAXE), stops(MAXE);
   for (int k = 0; k < MAXE; ++k) {
     HIPCHK(hipEventCreate(&starts[k]));
     HIPCHK(hipEventCreate(&stops[k]));
@@ -649,36 +679,47 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
   TestDesc* d = nullptr;
   std::vector<uint32_t> one(1, 1);
   if ((rc = t->group.upload_descs({t->obj}, &one, t->group.transient, &d))) return rc;
-  const char* names[MAXE] = {nullptr};
-  int nk = 0;
-  std::vector<double> acc(MAXE, 0.0);
-  for (uint64_t it = 0; it < iters; ++it) {
-    if (t->obj->cls->P.small) {
-      HIPCHK(launch_small_events(t->obj->cls->P, d, 1, t->group.stream, starts[0], stops[0]));
+  const DevPlan& P = t->obj->cls->P;
+  const bool fused = t->group.use_fused(1);
+  std::vector<std::string> order;
+  std::map<std::string, std::pair<double, uint64_t>> acc;
+  for (uint64_t done = 0; done < iters;) {
+    const char* names[MAXE] = {nullptr};
+    int nk = 0;
+    uint64_t k = 1;
+    if (P.small) {
+      HIPCHK(launch_small_events(P, d, 1, t->group.stream, starts[0], stops[0]));
       nk = 1;
       names[0] = "k_small";
+    } else if (fused) {
+      k = std::min<uint64_t>(RUN, iters - done);
+      HIPCHK(launch_run_fused_events(P, d, k, t->group.stream, starts.data(), stops.data(), MAXE, &nk, names));
     } else {
-      HIPCHK(launch_iteration_events(t->obj->cls->P, d, 1, t->group.stream, starts, stops, MAXE, &nk, names));
+      HIPCHK(launch_iteration_events(P, d, 1, t->group.stream, starts.data(), stops.data(), MAXE, &nk, names));
     }
-    ++t->obj->iter;
+    done += k;
+    t->obj->iter += k;
     HIPCHK(hipEventSynchronize(stops[nk - 1]));
-    for (int k = 0; k < nk; ++k) {
+    for (int j = 0; j < nk; ++j) {
       float f = 0;
-      HIPCHK(hipEventElapsedTime(&f, starts[k], stops[k]));
-      acc[k] += f;
+      HIPCHK(hipEventElapsedTime(&f, starts[j], stops[j]));
+      const std::string nm = names[j] ? names[j] : "?";
+      if (!acc.count(nm)) order.push_back(nm);
+      acc[nm].first += f;
+      acc[nm].second += 1;
     }
   }
   for (int k = 0; k < MAXE; ++k) {
     (void)hipEventDestroy(starts[k]);
     (void)hipEventDestroy(stops[k]);
   }
-  if ((rc = t->group.sync())) return rc;
-  const int m = std::min(nk, max_kernels);
+  if ((rc = test_sync(t))) return rc;
+  const int m = std::min((int)order.size(), max_kernels);
   std::string nm;
   for (int k = 0; k < m; ++k) {
-    ms[k] = acc[k];
-    nm += names[k] ? names[k] : "?";
-    nm += "\n";
+    const auto& a = acc[order[k]];
+    ms[k] = a.first / (double)a.second;
+    nm += order[k] + "\n";
   }
   *count = m;
   if (names_buf && names_len) snprintf(names_buf, names_len, "%s", nm.c_str());
@@ -753,7 +794,7 @@ static int ckpt_read(const char* path, CkptData& c) {
 int m2_checkpoint_save(m2_test* t, const char* path) {
   if (!t || !path) return fail(M2_E_INVAL, "null pointer");
   std::vector<uint8_t> r;
-  int rc = t->group.sync();
+  int rc = test_sync(t);
   if (rc) return rc;
   if ((rc = test_obj_residue(t->obj, r))) return rc;
   std::vector<uint8_t> b;

@@ -23,6 +23,10 @@ struct TestDesc {
   DigitGen dgen;
   int32_t mode;    // 0 = LL, 1 = PRP
   uint32_t iters;  // single-workgroup plan: iterations to run in this launch
+  // Fused outer pass (single tests whose strips fit one workgroup per CU):
+  // sync[c] counts the launches in which strip c has published its carries,
+  // sync[NS] is set if a wait for a neighbour timed out.
+  uint32_t* sync;
 };
 
 // Plan and tables of one transform length (shared by every test of that length).
@@ -87,6 +91,15 @@ inline uint64_t carry_target_digit(const DevPlan& P, uint64_t g) {
 // ---- launchers (m2_kernels.hip); `tests` is a device array of ntests descriptors ----
 hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st);
 hipError_t launch_iteration(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st);
+// k iterations of one test with the outer passes of consecutive iterations
+// fused: outer_fwd, (centre, outer_fused) x (k - 1), centre, outer_inv.
+hipError_t launch_run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, hipStream_t st);
+// The same with every launch's dispatch timestamps (at most max_events launches).
+hipError_t launch_run_fused_events(const DevPlan& P, const TestDesc* tests, uint64_t k, hipStream_t st,
+                                   hipEvent_t* starts, hipEvent_t* stops, int max_events, int* count,
+                                   const char** names);
+// Number of outer-pass strips (workgroups per outer launch) of plan P.
+inline uint32_t outer_strips(const DevPlan& P) { return (1u << P.logS) >> P.logW; }
 // One iteration with each kernel's dispatch timestamps in starts[k]/stops[k].
 hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
                                    hipEvent_t* starts, hipEvent_t* stops, int max_events, int* count,

@@ -652,6 +652,109 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) gst(dst + i, L61[PADX(i)]);
 }
 
+// The inverse outer pass of one iteration and the forward outer pass of the
+// next in one launch, for a single test whose NS strips fit one workgroup per
+// CU: the digits stay in LDS in between. The carry entering a strip row comes
+// from the left neighbour strip (or, for strip 0, the last strip one row up),
+// so after publishing its own row carries each workgroup waits only for that
+// one neighbour's, through a per-strip launch counter. Carries and counters
+// are written and read with device-scope (relaxed) atomics, which go to the
+// coherent level instead of an XCD's own L2; every thread waits for its carry
+// stores to complete before the workgroup raises its counter, and the
+// carries are read only after the neighbour's counter is seen raised, so no
+// cache-wide write-back or invalidation is needed. The wait is bounded: after
+// spin_limit polls it gives up and raises sync[NS], which the host reports.
+__device__ __forceinline__ void carry_publish(M2_GLOBAL Carry128* G, uint32_t i, const Carry128& c) {
+  __hip_atomic_store((uint64_t*)&G[i].lo, c.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((int64_t*)&G[i].hi, c.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ Carry128 carry_fetch(const M2_GLOBAL Carry128* G, uint32_t i) {
+  Carry128 c;
+  c.lo = __hip_atomic_load((const uint64_t*)&G[i].lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  c.hi = __hip_atomic_load((const int64_t*)&G[i].hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return c;
+}
+
+__global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __restrict__ tests,
+                                                     const e61* __restrict__ tw61, const e31* __restrict__ tw31,
+                                                     Pass ps, int logS, int logW, uint32_t spin_limit) {
+  __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
+  __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
+  __shared__ e61 T61[OUTER_TW];
+  __shared__ e31 T31[OUTER_TW];
+  __shared__ Carry128 gs[PASS_MAX_ELEMS / SEG_ELEMS];
+  __shared__ uint32_t epoch;
+  const TestDesc td = tests[blockIdx.y];
+  M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
+  M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
+  M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
+  M2_GLOBAL uint32_t* gsync = gptr(td.sync);
+  uint32_t* sync = (uint32_t*)gsync;  // generic pointer to global memory, for the atomic builtins
+  const DigitGen D = td.dgen;
+  const int mode = td.mode;
+  const int R = 1 << ps.A, W = 1 << logW;
+  const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
+  const uint32_t col0 = cg << logW;
+  const uint32_t j0[1] = {0};
+  if (threadIdx.x == 0) epoch = __hip_atomic_load(sync + cg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+  gather2<4>(L61, L31, z61, z31, R * W,
+             [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); },
+             [](int i) { return PADX(i); });
+  stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
+  __syncthreads();
+  // inverse outer pass, unweight + CRT, segment carries (as k_outer_inv)
+  lds_pass<false, true>(L61, L31, logW, 1, ps, T61, T31);
+  const int subs = W / SEG_ELEMS;
+  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
+    const int r = it / subs, v = it - r * subs;
+    const int idx0 = r * W + v * SEG_ELEMS;
+    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
+    const i128 c0 = (mode == 0 && e0 == 0) ? (i128)(-2) : (i128)0;
+    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0));
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
+    const int r = it / subs, v = it - r * subs;
+    const int idx0 = r * W + v * SEG_ELEMS;
+    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
+    if (v > 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gs, it - 1));
+    if (v == subs - 1) carry_publish(G, cg * (uint32_t)R + (uint32_t)r, gs[it]);
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's carry stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(sync + cg, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t prod = cg ? cg - 1u : NS - 1u;
+    uint32_t n = 0;
+    while (__hip_atomic_load(sync + prod, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+      if (++n >= spin_limit) {
+        __hip_atomic_store(sync + NS, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+  // carry entering each strip row, then weight and the forward outer pass
+  // (as k_outer_fwd; gs is reused for the incoming carries)
+  for (int rr = threadIdx.x; rr < R; rr += blockDim.x) gs[rr] = carry_fetch(G, prev_strip_carry(cg, rr, R, NS));
+  __syncthreads();
+  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
+    const int r = it / subs, v = it - r * subs;
+    const int idx0 = r * W + v * SEG_ELEMS;
+    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
+    if (v == 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gs, (uint32_t)r));
+    seg_weight(L61, L31, idx0, 2u * e0, D);
+  }
+  __syncthreads();
+  lds_pass<true, true>(L61, L31, logW, 1, ps, T61, T31);
+  for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
+    const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
+    gst(z61 + e, L61[PADX(i)]);
+    gst(z31 + e, L31[PADX(i)]);
+  }
+}
+
 // Middle pass: levels [ell0, ell0 + A) with no carry work. Level-ell0 block
 // j (of R * S elements) is split into S/W column groups.
 template <bool FWD>
@@ -787,7 +890,59 @@ hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& l
     return e;
   return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer, nt), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
 }
+
+// Polls of a neighbour's launch counter before the fused pass gives up
+// (a poll is a device-scope load plus a short sleep, about a microsecond:
+// about a second in all).
+constexpr uint32_t FUSED_SPIN_LIMIT = 1u << 20;
+
+// k >= 1 iterations of a single test (no middle pass) with the outer passes of
+// consecutive iterations fused: the run starts and ends with the digits.
+template <class F>
+hipError_t run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& launch) {
+  hipError_t e;
+  const uint32_t ns = outer_strips(P);
+  const uint32_t ncentre = (P.h >> P.centre.A) / 2;
+  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
+      hipSuccess)
+    return e;
+  for (uint64_t i = 0; i + 1 < k; ++i) {
+    if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre)) !=
+        hipSuccess)
+      return e;
+    if ((e = launch("k_outer_fused", k_outer_fused, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW,
+                    FUSED_SPIN_LIMIT)) != hipSuccess)
+      return e;
+  }
+  if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre)) != hipSuccess)
+    return e;
+  return launch("k_outer_inv", k_outer_inv, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
+}
 }  // namespace
+
+hipError_t launch_run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, hipStream_t st) {
+  if (P.small || P.has_mid || k == 0) return hipErrorInvalidValue;
+  return run_fused(P, tests, k, [&](const char*, auto kernel, dim3 grid, auto... args) -> hipError_t {
+    hipLaunchKernelGGL(kernel, grid, dim3(MP_THREADS), 0, st, args...);
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_run_fused_events(const DevPlan& P, const TestDesc* tests, uint64_t k, hipStream_t st,
+                                   hipEvent_t* starts, hipEvent_t* stops, int max_events, int* count,
+                                   const char** names) {
+  if (P.small || P.has_mid || k == 0) return hipErrorInvalidValue;
+  int n = 0;
+  hipError_t e = run_fused(P, tests, k, [&](const char* name, auto kernel, dim3 grid, auto... args) -> hipError_t {
+    if (n >= max_events) return hipErrorInvalidValue;
+    names[n] = name;
+    hipExtLaunchKernelGGL(kernel, grid, dim3(MP_THREADS), 0, st, starts[n], stops[n], 0, args...);
+    ++n;
+    return hipGetLastError();
+  });
+  *count = n;
+  return e;
+}
 
 hipError_t launch_iteration(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
   return iteration(P, tests, ntests, [&](const char*, auto kernel, dim3 grid, auto... args) -> hipError_t {

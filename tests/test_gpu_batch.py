@@ -187,3 +187,19 @@ def test_autotune_and_tuned_batch(gpu_ctx):
         b.run(50)
         for i, c in enumerate(cases):
             assert b.res64(i) == c["checkpoints"]["50"]["res64"], (c["p"], c["mode"])
+
+
+def test_fused_and_separate_outer_passes_agree(gpu_ctx):
+    """Single tests fuse the outer passes of consecutive iterations where the
+    plan allows; the separate launches give the same residues."""
+    for c in [c for c in _sweep_cases() if 13 <= c["ln"] <= 17 and c["mode"] == "prp"][:4]:
+        with m2.Test(gpu_ctx, c["p"], "prp", local_stage_cap=64) as a, \
+                m2.Test(gpu_ctx, c["p"], "prp", local_stage_cap=64, no_fuse=1) as b:
+            for k in (5, 50, 200):
+                a.run(k - a.iteration)
+                b.run(k - b.iteration)
+                assert a.res64() == b.res64() == c["checkpoints"][str(k)]["res64"], (c["p"], k)
+    c2 = load_fixture("c2.json")["cases"][0]
+    with m2.Test(gpu_ctx, c2["p"], "prp", no_fuse=1) as b:
+        b.run(100)
+        assert b.res64() == c2["checkpoints"]["100"]["res64"]
