@@ -114,6 +114,14 @@ int m2_test_kernel_times(m2_test *t, uint64_t iters, double *ms, int max_kernels
 /* Human-readable plan (the reference's --debug-plan, mersenne2_gpu.cpp:4767-4776). */
 int m2_test_plan(m2_test *t, char *buf, size_t len);
 
+/* Test hook: the number of polls the fused outer pass makes while waiting
+ * for a neighbouring workgroup's carries before it gives up and reports
+ * M2_E_STATE at the next sync (default 2^20, about a second). 0 reports the
+ * time-out without waiting, so the reporting path can be exercised without a
+ * real stall. Applies to launches made afterwards (a test's captured graphs
+ * keep the value they were made with). */
+int m2_debug_fused_spin_limit(uint32_t polls, uint32_t *previous);
+
 /* Checkpoints: the canonical residue with p, mode, iteration and Res64 in a
  * small checksummed file (absent in the reference, which cannot stop and
  * resume a test). Save writes a temporary file and renames it over `path`.

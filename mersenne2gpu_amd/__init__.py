@@ -48,6 +48,7 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "m2_test_kernel_times": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double), _c.c_int, _c.POINTER(_c.c_int),
                                         _c.c_char_p, _c.c_size_t]),
     "m2_test_plan": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
+    "m2_debug_fused_spin_limit": (_c.c_int, [_c.c_uint32, _c.POINTER(_c.c_uint32)]),
     "m2_checkpoint_save": (_c.c_int, [_P, _c.c_char_p]),
     "m2_checkpoint_load": (_c.c_int, [_P, _c.c_char_p]),
     "m2_checkpoint_info": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),

@@ -726,6 +726,12 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
   return M2_OK;
 }
 
+int m2_debug_fused_spin_limit(uint32_t polls, uint32_t* previous) {
+  if (previous) *previous = g_fused_spin_limit;
+  g_fused_spin_limit = polls;
+  return M2_OK;
+}
+
 // ---- checkpoints ----
 // File layout (little-endian): magic "M2CKPT01", u32 p, u32 mode, u64
 // iteration, u64 Res64, u64 n = ceil(p/8), n residue bytes (canonical R),

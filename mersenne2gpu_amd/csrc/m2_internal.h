@@ -98,6 +98,9 @@ hipError_t launch_run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k,
 hipError_t launch_run_fused_events(const DevPlan& P, const TestDesc* tests, uint64_t k, hipStream_t st,
                                    hipEvent_t* starts, hipEvent_t* stops, int max_events, int* count,
                                    const char** names);
+// Poll limit of the fused pass's wait for a neighbour (launches made after a
+// change use the new value; captured graphs keep the value they were made with).
+extern uint32_t g_fused_spin_limit;
 // Number of outer-pass strips (workgroups per outer launch) of plan P.
 inline uint32_t outer_strips(const DevPlan& P) { return (1u << P.logS) >> P.logW; }
 // One iteration with each kernel's dispatch timestamps in starts[k]/stops[k].

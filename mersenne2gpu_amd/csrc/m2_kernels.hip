@@ -726,6 +726,9 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
     __hip_atomic_store(sync + cg, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t prod = cg ? cg - 1u : NS - 1u;
     uint32_t n = 0;
+    // spin_limit 0: report a time-out without waiting (the host's way to
+    // exercise the report path)
+    if (spin_limit == 0) __hip_atomic_store(sync + NS, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (__hip_atomic_load(sync + prod, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
       if (++n >= spin_limit) {
         __hip_atomic_store(sync + NS, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -868,6 +871,11 @@ hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests
   return hipGetLastError();
 }
 
+// Polls of a neighbour's launch counter before the fused pass gives up
+// (a poll is a device-scope load plus a short sleep, about a microsecond:
+// about a second in all). Settable for tests (m2_debug_fused_spin_limit).
+uint32_t g_fused_spin_limit = 1u << 20;
+
 namespace {
 // Launches the kernels of one iteration of every test in `tests` through
 // launch(name, kernel, grid, args...); grid.y indexes the tests.
@@ -891,10 +899,6 @@ hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& l
   return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer, nt), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
 }
 
-// Polls of a neighbour's launch counter before the fused pass gives up
-// (a poll is a device-scope load plus a short sleep, about a microsecond:
-// about a second in all).
-constexpr uint32_t FUSED_SPIN_LIMIT = 1u << 20;
 
 // k >= 1 iterations of a single test (no middle pass) with the outer passes of
 // consecutive iterations fused: the run starts and ends with the digits.
@@ -911,7 +915,7 @@ hipError_t run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& la
         hipSuccess)
       return e;
     if ((e = launch("k_outer_fused", k_outer_fused, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW,
-                    FUSED_SPIN_LIMIT)) != hipSuccess)
+                    g_fused_spin_limit)) != hipSuccess)
       return e;
   }
   if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre)) != hipSuccess)

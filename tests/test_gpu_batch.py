@@ -203,3 +203,26 @@ def test_fused_and_separate_outer_passes_agree(gpu_ctx):
     with m2.Test(gpu_ctx, c2["p"], "prp", no_fuse=1) as b:
         b.run(100)
         assert b.res64() == c2["checkpoints"]["100"]["res64"]
+
+
+def test_fused_pass_reports_a_wait_time_out(gpu_ctx):
+    """The fused outer pass's bounded wait: with the poll limit forced to 0
+    from the host, every fused launch reports a time-out (without waiting)
+    and the next sync fails with M2_E_STATE; with the limit restored, a fresh
+    test runs exactly. Direct launches only (graph_iters=1), so no captured
+    graph keeps the forced value."""
+    import ctypes
+    c2 = load_fixture("c2.json")["cases"][0]
+    prev = ctypes.c_uint32()
+    m2.lib().m2_debug_fused_spin_limit(0, ctypes.byref(prev))
+    try:
+        with m2.Test(gpu_ctx, c2["p"], "prp", graph_iters=1) as t:
+            t.run(3)
+            with pytest.raises(m2.M2Error) as ei:
+                t.sync()
+            assert ei.value.code == -6
+    finally:
+        m2.lib().m2_debug_fused_spin_limit(prev.value, None)
+    with m2.Test(gpu_ctx, c2["p"], "prp", graph_iters=1) as t:
+        t.run(10)
+        assert t.res64() == c2["checkpoints"]["10"]["res64"]
