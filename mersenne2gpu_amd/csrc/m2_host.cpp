@@ -137,7 +137,11 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
   }
   if (L <= SMALL_LOG_H && !(ls_cap && L >= c + 2)) {
     P.small = true;
-    P.outer = make_pass(0, L, T.tw61, T.tw31, maxr);
+    // A single workgroup's transform is bound by latency: radix-4 steps
+    // measured faster than radix-8 up to h = 1024 (13.2 vs 15.4 us/iter at
+    // n = 512, 17.3 vs 20.1 at n = 2048), radix-8 faster at h = 2048
+    // (25.4 vs 30.5 at n = 4096).
+    P.outer = make_pass(0, L, T.tw61, T.tw31, opts.max_radix ? maxr : (L <= 10 ? 4 : 8));
     P.nseg = (uint32_t)(h / SEG_ELEMS);
   } else {
     if (L < c + 2) {

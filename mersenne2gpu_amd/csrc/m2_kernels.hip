@@ -493,10 +493,14 @@ __device__ __forceinline__ void stage_range(e61* S61, e31* S31, const e61* __res
   }
 }
 
+// Threads of the single-workgroup kernel: 256 per field (M61 and M31 waves
+// side by side, as in the multi-pass kernels).
+constexpr int SMALL_THREADS = 512;
+
 // ---------------------------------------------------------------------------
 // Small plan: one workgroup holds the whole residue (h <= 2^SMALL_LOG_H) and
 // runs `iters` iterations without leaving LDS. Digits are in natural order.
-__global__ void __launch_bounds__(256) k_small(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+__global__ void __launch_bounds__(SMALL_THREADS) k_small(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                                const e31* __restrict__ tw31, const e61* __restrict__ sq61,
                                                const e31* __restrict__ sq31, Pass ps) {
   __shared__ e61 L61[padded(1 << SMALL_LOG_H)];
@@ -520,9 +524,9 @@ __global__ void __launch_bounds__(256) k_small(const TestDesc* __restrict__ test
   for (uint32_t it = 0; it < td.iters; ++it) {
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_weight(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D);
     __syncthreads();
-    lds_pass<true, false>(L61, L31, 0, 1, ps, T61, T31);
-    square_self<false>(L61, L31, logH, 1, S61, S31);
-    lds_pass<false, false>(L61, L31, 0, 1, ps, T61, T31);
+    lds_pass<true, true>(L61, L31, 0, 1, ps, T61, T31);
+    square_self<true>(L61, L31, logH, 1, S61, S31);
+    lds_pass<false, true>(L61, L31, 0, 1, ps, T61, T31);
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
       const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
       store_carry(gs, s, seg_normalise(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D, c0));
@@ -867,7 +871,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
 // Host-side launchers, used by the engine.
 
 hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
-  hipLaunchKernelGGL(k_small, dim3(ntests), dim3(256), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
+  hipLaunchKernelGGL(k_small, dim3(ntests), dim3(SMALL_THREADS), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31,
+                     P.outer);
   return hipGetLastError();
 }
 
@@ -972,7 +977,7 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
 
 hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
                                hipEvent_t start, hipEvent_t stop) {
-  hipExtLaunchKernelGGL(k_small, dim3(ntests), dim3(256), 0, st, start, stop, 0, tests, P.tw61, P.tw31, P.sq61,
+  hipExtLaunchKernelGGL(k_small, dim3(ntests), dim3(SMALL_THREADS), 0, st, start, stop, 0, tests, P.tw61, P.tw31, P.sq61,
                         P.sq31, P.outer);
   return hipGetLastError();
 }
