@@ -44,6 +44,26 @@ constexpr uint32_t GRAPH_ITERS = 16;    // iterations per captured graph (multi-
 
 uint64_t full_iters(uint32_t p, int mode) { return mode == M2_MODE_LL ? (p >= 2 ? p - 2 : 0) : p; }
 
+// Plan for a single test on gfx950 when the caller leaves the plan fields at
+// zero (the counterpart of the reference's per-device profile table,
+// mersenne2_gpu.cpp:3515-3616): m2_autotune's choices for one test, measured
+// with the fused outer pass (us/iteration, default plan -> chosen):
+//   n = 2^13: 39.1 -> 30.0 (centre 2^6, carry block 16)   2^14: 40.2 -> 32.3 (2^6, 16)
+//   n = 2^15: 44.0 -> 34.0 (centre 2^8, carry block 16)   2^16: 44.7 -> 36.4 (2^8, 16)
+//   n = 2^17: 46.1 -> 40.0 (centre 2^6)                   2^18: 50.0 -> 44.4 (2^8)
+//   n = 2^19: 53.8 -> 48.7 (centre 2^8)                   others: the default plan.
+PlanOpts single_test_profile(int ln) {
+  PlanOpts r;
+  switch (ln) {
+    case 13: case 14: r.ls_cap = 128; r.carry_pairs = 16; break;
+    case 15: case 16: r.ls_cap = 512; r.carry_pairs = 16; break;
+    case 17: r.ls_cap = 128; break;
+    case 18: case 19: r.ls_cap = 512; break;
+    default: break;
+  }
+  return r;
+}
+
 PlanOpts plan_opts(const m2_opts* o) {
   PlanOpts r;
   if (o) {
@@ -535,11 +555,18 @@ int m2_test_create(m2_ctx* ctx, uint64_t p, int mode, const m2_opts* opts, m2_te
   if (opts && opts->graph_iters) t->graph_iters = opts->graph_iters;
   const int ln = (p >= 3 && p <= 0xFFFFFFFFull) ? transform_log2(p) : 0;
   if (ln < 3 || ln > 27) return fail(M2_E_RANGE, "exponent out of range (3 <= p < 2^32, n <= 2^27)");
+  PlanOpts po = plan_opts(opts);
+  if (!po.ls_cap && !po.carry_pairs && !po.max_radix) po = single_test_profile(ln);
+  // an exponent whose digits are too narrow for the profile's multi-pass plan keeps the default
+  {
+    const uint32_t q = (uint32_t)(p >> ln);
+    if (po.ls_cap && q < 12) po = plan_opts(opts);
+  }
   Class* cls = nullptr;
-  int rc = get_class(ctx, ln, plan_opts(opts), &cls);
+  int rc = get_class(ctx, ln, po, &cls);
   if (rc) return rc;
   if ((rc = t->group.create(cls))) return rc;
-  if ((rc = test_obj_create(ctx, p, mode, plan_opts(opts), t->group.stream, &t->obj))) {
+  if ((rc = test_obj_create(ctx, p, mode, po, t->group.stream, &t->obj))) {
     t->group.destroy();
     return rc;
   }
@@ -890,6 +917,7 @@ int time_candidate(m2_ctx* ctx, uint64_t p, uint32_t ntests, const m2_opts& o, d
   Group g;
   if ((rc = g.create(cls))) return rc;
   g.graph_iters = o.graph_iters ? o.graph_iters : GRAPH_ITERS;
+  g.fused = ntests == 1 && !o.no_fuse;  // a single test runs as m2_test would
   std::vector<TestObj*> ts;
   auto cleanup = [&](int code) {
     g.destroy();
