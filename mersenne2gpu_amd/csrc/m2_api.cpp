@@ -64,6 +64,13 @@ PlanOpts single_test_profile(int ln) {
   return r;
 }
 
+// Iterations per captured graph for a single test, by length: each graph
+// starts with a forward and ends with an inverse outer pass instead of a fused
+// one, and graph launches leave short gaps, so long graphs pay where an
+// iteration is long (n = 2^20: 59.1 us/iteration at 16, 58.5 at 64, 58.3 at
+// 256, measured on the MI355X).
+uint32_t single_test_graph_iters(int ln) { return ln >= 17 ? 256u : (ln >= 13 ? 64u : GRAPH_ITERS); }
+
 PlanOpts plan_opts(const m2_opts* o) {
   PlanOpts r;
   if (o) {
@@ -227,6 +234,10 @@ int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, const PlanOpts& po, hipSt
     if ((rc = dev_alloc(t->allocs, t->bytes, &d, nullptr, nsync))) return bail(rc);
     t->desc.sync = (uint32_t*)d;
     HIPCHK(hipMemsetAsync(d, 0, nsync, st));
+    const size_t ncf = (size_t)P.nseg * 2 * sizeof(uint64_t);
+    if ((rc = dev_alloc(t->allocs, t->bytes, &d, nullptr, ncf))) return bail(rc);
+    t->desc.cf = (uint64_t*)d;
+    HIPCHK(hipMemsetAsync(d, 0, ncf, st));
   }
   t->desc.dgen = t->dgen;
   t->desc.mode = mode;
@@ -552,9 +563,9 @@ int m2_test_create(m2_ctx* ctx, uint64_t p, int mode, const m2_opts* opts, m2_te
   HIPCHK(hipSetDevice(ctx->device));
   std::unique_ptr<m2_test> t(new m2_test);
   t->ctx = ctx;
-  if (opts && opts->graph_iters) t->graph_iters = opts->graph_iters;
   const int ln = (p >= 3 && p <= 0xFFFFFFFFull) ? transform_log2(p) : 0;
   if (ln < 3 || ln > 27) return fail(M2_E_RANGE, "exponent out of range (3 <= p < 2^32, n <= 2^27)");
+  t->graph_iters = (opts && opts->graph_iters) ? opts->graph_iters : single_test_graph_iters(ln);
   PlanOpts po = plan_opts(opts);
   if (!po.ls_cap && !po.carry_pairs && !po.max_radix) po = single_test_profile(ln);
   // an exponent whose digits are too narrow for the profile's multi-pass plan keeps the default

@@ -24,9 +24,11 @@ struct TestDesc {
   int32_t mode;    // 0 = LL, 1 = PRP
   uint32_t iters;  // single-workgroup plan: iterations to run in this launch
   // Fused outer pass (single tests whose strips fit one workgroup per CU):
-  // sync[c] counts the launches in which strip c has published its carries,
-  // sync[NS] is set if a wait for a neighbour timed out.
+  // sync[c] counts strip c's launches, sync[NS] is set if a wait for a
+  // neighbour timed out; cf holds the strip-row carries as tagged words (two
+  // per strip row, tag = launch count mod 256).
   uint32_t* sync;
+  uint64_t* cf;
 };
 
 // Plan and tables of one transform length (shared by every test of that length).

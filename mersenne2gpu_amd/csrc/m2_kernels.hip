@@ -245,7 +245,10 @@ struct Lanes {
   }
 };
 
-template <bool FWD, int RADIX, bool SPLIT>
+// TWIST = false is the step at level 0, whose only block has tau = 1: no
+// twiddle products; the inverse then reduces its M61 outputs itself (the
+// carry step's unweighting rotates 61-bit values).
+template <bool FWD, int RADIX, bool SPLIT, bool TWIST = true>
 __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW, int nb, int lam, const e61* Ts61,
                                          const e31* Ts31) {
   constexpr int LR = RADIX == 8 ? 3 : (RADIX == 4 ? 2 : 1);
@@ -271,8 +274,10 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
 #pragma unroll
       for (int e = 0; e < RADIX; ++e) v[e] = L61[PADX(base + e * stride)];
       if (FWD) {
+        if (TWIST) {
 #pragma unroll
-        for (int e = 1; e < RADIX; ++e) v[e] = mul_tw(v[e], T61[e - 1]);
+          for (int e = 1; e < RADIX; ++e) v[e] = mul_tw(v[e], T61[e - 1]);
+        }
         if (RADIX == 8) dft8_fwd_lazy(v);
         if (RADIX == 4) dft4_fwd_lazy(v);
         if (RADIX == 2) dft2_fwd_lazy(v[0], v[1]);
@@ -280,8 +285,13 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
         if (RADIX == 8) dft8_inv_lazy(v);
         if (RADIX == 4) dft4_inv_lazy(v);
         if (RADIX == 2) dft2_inv_lazy(v[0], v[1]);
+        if (TWIST) {
 #pragma unroll
-        for (int e = 1; e < RADIX; ++e) v[e] = mulconj_tw(v[e], T61[e - 1]);
+          for (int e = 1; e < RADIX; ++e) v[e] = mulconj_tw(v[e], T61[e - 1]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < RADIX; ++e) v[e] = {red61(v[e].a), red61(v[e].b)};
+        }
       }
 #pragma unroll
       for (int e = 0; e < RADIX; ++e) L61[PADX(base + e * stride)] = v[e];
@@ -292,8 +302,10 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
 #pragma unroll
       for (int e = 0; e < RADIX; ++e) x[e] = L31[PADX(base + e * stride)];
       if (FWD) {
+        if (TWIST) {
 #pragma unroll
-        for (int e = 1; e < RADIX; ++e) x[e] = mul_tw(x[e], T31[e - 1]);
+          for (int e = 1; e < RADIX; ++e) x[e] = mul_tw(x[e], T31[e - 1]);
+        }
         if (RADIX == 8) dft8_fwd(x);
         if (RADIX == 4) dft4_fwd(x);
         if (RADIX == 2) dft2(x[0], x[1]);
@@ -301,8 +313,10 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
         if (RADIX == 8) dft8_inv(x);
         if (RADIX == 4) dft4_inv(x);
         if (RADIX == 2) dft2(x[0], x[1]);
+        if (TWIST) {
 #pragma unroll
-        for (int e = 1; e < RADIX; ++e) x[e] = mulconj_tw(x[e], T31[e - 1]);
+          for (int e = 1; e < RADIX; ++e) x[e] = mulconj_tw(x[e], T31[e - 1]);
+        }
       }
 #pragma unroll
       for (int e = 0; e < RADIX; ++e) L31[PADX(base + e * stride)] = x[e];
@@ -312,8 +326,9 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
 }
 
 // All steps of a pass, forward (in order) or inverse (reversed), twiddles in
-// LDS as laid out by stage_pass_twiddles.
-template <bool FWD, bool SPLIT>
+// LDS as laid out by stage_pass_twiddles. LEVEL0: the pass may start at level
+// 0, whose step runs without twiddle products.
+template <bool FWD, bool SPLIT, bool LEVEL0 = false>
 __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, const Pass& ps, const e61* T61,
                                          const e31* T31, int k0 = 0, int k1 = MAX_STEPS) {
   int offs[MAX_STEPS];
@@ -326,6 +341,12 @@ __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, c
     const int si = FWD ? k : ps.nsteps - 1 - k;
     const Step st = ps.steps[si];
     const int lam = st.ell - ps.ell0;
+    if (LEVEL0 && st.ell == 0) {
+      if (st.radix == 8) lds_step<FWD, 8, SPLIT, false>(L61, L31, ps.A, logW, nb, lam, T61, T31);
+      else if (st.radix == 4) lds_step<FWD, 4, SPLIT, false>(L61, L31, ps.A, logW, nb, lam, T61, T31);
+      else lds_step<FWD, 2, SPLIT, false>(L61, L31, ps.A, logW, nb, lam, T61, T31);
+      continue;
+    }
     if (st.radix == 8) lds_step<FWD, 8, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
     else if (st.radix == 4) lds_step<FWD, 4, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
     else lds_step<FWD, 2, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
@@ -524,9 +545,9 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_small(const TestDesc* __restr
   for (uint32_t it = 0; it < td.iters; ++it) {
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_weight(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D);
     __syncthreads();
-    lds_pass<true, true>(L61, L31, 0, 1, ps, T61, T31);
+    lds_pass<true, true, true>(L61, L31, 0, 1, ps, T61, T31);
     square_self<true>(L61, L31, logH, 1, S61, S31);
-    lds_pass<false, true>(L61, L31, 0, 1, ps, T61, T31);
+    lds_pass<false, true, true>(L61, L31, 0, 1, ps, T61, T31);
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
       const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
       store_carry(gs, s, seg_normalise(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D, c0));
@@ -603,7 +624,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
     seg_weight(L61, L31, idx0, 2u * e0, D);
   }
   __syncthreads();
-  lds_pass<true, true>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
     gst(z61 + e, L61[PADX(i)]);
@@ -634,7 +655,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
              [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
-  lds_pass<false, true>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31);
   const int subs = W / SEG_ELEMS;
   for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
     const int r = it / subs, v = it - r * subs;
@@ -659,24 +680,37 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
 // The inverse outer pass of one iteration and the forward outer pass of the
 // next in one launch, for a single test whose NS strips fit one workgroup per
 // CU: the digits stay in LDS in between. The carry entering a strip row comes
-// from the left neighbour strip (or, for strip 0, the last strip one row up),
-// so after publishing its own row carries each workgroup waits only for that
-// one neighbour's, through a per-strip launch counter. Carries and counters
-// are written and read with device-scope (relaxed) atomics, which go to the
-// coherent level instead of an XCD's own L2; every thread waits for its carry
-// stores to complete before the workgroup raises its counter, and the
-// carries are read only after the neighbour's counter is seen raised, so no
-// cache-wide write-back or invalidation is needed. The wait is bounded: after
-// spin_limit polls it gives up and raises sync[NS], which the host reports.
-__device__ __forceinline__ void carry_publish(M2_GLOBAL Carry128* G, uint32_t i, const Carry128& c) {
-  __hip_atomic_store((uint64_t*)&G[i].lo, c.lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store((int64_t*)&G[i].hi, c.hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// from the left neighbour strip (or, for strip 0, the last strip one row up).
+// Each strip row's carry is published as two 64-bit words, each holding 56
+// bits of the value under an 8-bit tag, the launch count mod 256; a 64-bit
+// store is single-copy atomic, so a reader that sees both words tagged with
+// its own launch's count has the whole value, with no flag, fence or cache
+// write-back between writer and reader. The words are written and read with
+// device-scope (relaxed) atomics, which go to the coherent level instead of
+// an XCD's own L2. The wait is bounded: after spin_limit polls a reader gives
+// up and raises sync[NS], which the host reports.
+__device__ __forceinline__ void carry_put(M2_GLOBAL uint64_t* cf, uint32_t i, i128 c, uint32_t tag) {
+  const uint64_t t = (uint64_t)tag << 56, m = (1ull << 56) - 1ull;
+  uint64_t* w = (uint64_t*)(cf + 2 * i);
+  __hip_atomic_store(w, ((uint64_t)c & m) | t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w + 1, ((uint64_t)(c >> 56) & m) | t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ Carry128 carry_fetch(const M2_GLOBAL Carry128* G, uint32_t i) {
-  Carry128 c;
-  c.lo = __hip_atomic_load((const uint64_t*)&G[i].lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  c.hi = __hip_atomic_load((const int64_t*)&G[i].hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return c;
+__device__ __forceinline__ i128 carry_take(M2_GLOBAL uint64_t* cf, uint32_t i, uint32_t tag, uint32_t limit,
+                                           uint32_t* fault) {
+  uint64_t* w = (uint64_t*)(cf + 2 * i);
+  for (uint32_t n = 0;;) {
+    const uint64_t w0 = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t w1 = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(w0 >> 56) == tag && (uint32_t)(w1 >> 56) == tag) {
+      const int64_t hi = (int64_t)(w1 << 8) >> 8;  // upper 56 bits, sign-extended
+      return ((i128)hi << 56) | (i128)(w0 & ((1ull << 56) - 1ull));
+    }
+    if (++n >= limit) {
+      __hip_atomic_store(fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return 0;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
 }
 
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __restrict__ tests,
@@ -691,7 +725,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   const TestDesc td = tests[blockIdx.y];
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
-  M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
+  M2_GLOBAL uint64_t* __restrict__ cf = gptr(td.cf);
   M2_GLOBAL uint32_t* gsync = gptr(td.sync);
   uint32_t* sync = (uint32_t*)gsync;  // generic pointer to global memory, for the atomic builtins
   const DigitGen D = td.dgen;
@@ -707,7 +741,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   // inverse outer pass, unweight + CRT, segment carries (as k_outer_inv)
-  lds_pass<false, true>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31);
   const int subs = W / SEG_ELEMS;
   for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
     const int r = it / subs, v = it - r * subs;
@@ -722,29 +756,19 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
     const int idx0 = r * W + v * SEG_ELEMS;
     const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
     if (v > 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gs, it - 1));
-    if (v == subs - 1) carry_publish(G, cg * (uint32_t)R + (uint32_t)r, gs[it]);
+    if (v == subs - 1) carry_put(cf, cg * (uint32_t)R + (uint32_t)r, load_carry(gs, it), epoch & 0xFFu);
   }
-  __builtin_amdgcn_s_waitcnt(0);  // this thread's carry stores have completed
   __syncthreads();
   if (threadIdx.x == 0) {
     __hip_atomic_store(sync + cg, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t prod = cg ? cg - 1u : NS - 1u;
-    uint32_t n = 0;
     // spin_limit 0: report a time-out without waiting (the host's way to
     // exercise the report path)
     if (spin_limit == 0) __hip_atomic_store(sync + NS, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(sync + prod, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-      if (++n >= spin_limit) {
-        __hip_atomic_store(sync + NS, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
   }
-  __syncthreads();
   // carry entering each strip row, then weight and the forward outer pass
   // (as k_outer_fwd; gs is reused for the incoming carries)
-  for (int rr = threadIdx.x; rr < R; rr += blockDim.x) gs[rr] = carry_fetch(G, prev_strip_carry(cg, rr, R, NS));
+  for (int rr = threadIdx.x; rr < R; rr += blockDim.x)
+    store_carry(gs, rr, carry_take(cf, prev_strip_carry(cg, rr, R, NS), epoch & 0xFFu, spin_limit, sync + NS));
   __syncthreads();
   for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
     const int r = it / subs, v = it - r * subs;
@@ -754,7 +778,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
     seg_weight(L61, L31, idx0, 2u * e0, D);
   }
   __syncthreads();
-  lds_pass<true, true>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
     gst(z61 + e, L61[PADX(i)]);
