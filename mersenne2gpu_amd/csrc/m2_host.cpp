@@ -163,8 +163,8 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
       while ((2u << lw) <= opts.carry_pairs) ++lw;
       P.logW = std::max(2, std::min(lw, P.logW));
     }
-    if ((1 << P.logW) < SEG_ELEMS) {
-      err = "plan error: outer pass too tall";
+    if (a_outer > PASS_MAX_LOG_R || a_mid > PASS_MAX_LOG_R || (1 << P.logW) < SEG_ELEMS) {
+      err = "plan error: strided passes too tall (local stage cap too small for this length)";
       return false;
     }
     if (a_mid) {

@@ -588,7 +588,10 @@ __device__ __forceinline__ uint32_t xcd_strip(uint32_t b, uint32_t n) {
   return (n % NUM_XCD) ? b : (b % NUM_XCD) * (n / NUM_XCD) + b / NUM_XCD;
 }
 
-constexpr int OUTER_TW = PASS_MAX_ELEMS / 2;  // R - 1 twiddles per block, R <= 512 here
+// A strided pass over R rows stages R - 1 twiddles; with R <= 2^9 the outer
+// and middle passes fit two workgroups per CU (74 KB of LDS each), so one
+// workgroup's loads and stores overlap another's arithmetic.
+constexpr int OUTER_TW = 1 << PASS_MAX_LOG_R;
 
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                                    const e31* __restrict__ tw31, Pass ps, int logS, int logW) {

@@ -21,6 +21,7 @@ constexpr int MAX_STEPS = 8;
 constexpr int SMALL_LOG_H = 11;     // h <= 2048 elements (48 KB of data in LDS)
 constexpr int CENTRE_LOG_C = 10;    // centre block: 1024 elements, two per workgroup
 constexpr int PASS_MAX_ELEMS = 2048;  // R * W elements per strided-pass workgroup
+constexpr int PASS_MAX_LOG_R = 9;     // strided passes: at most 2^9 rows (R - 1 twiddles)
 constexpr int SEG_ELEMS = 4;        // carry segment: 4 elements = 8 digits
 
 // One radix step inside a pass: global level `ell`, radix 2/4/8, and the

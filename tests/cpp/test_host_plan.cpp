@@ -100,6 +100,29 @@ int main() {
       }
     }
   }
+  // Strided passes hold at most 2^PASS_MAX_LOG_R rows and PASS_MAX_ELEMS
+  // elements per workgroup (their LDS arrays are sized for that); a local
+  // stage cap that would need taller passes is refused, whatever carry_pairs.
+  for (uint32_t cp : {0u, 16u}) {
+    const struct { int ln; uint32_t cap; bool ok; } cases[] = {{22, 4, false}, {23, 8, false}, {22, 8, true}};
+    for (const auto& c : cases) {
+      DevPlan P;
+      ClassTables CT;
+      std::string err;
+      PlanOpts po;
+      po.ls_cap = c.cap;
+      po.carry_pairs = cp;
+      const bool ok = build_class_plan(c.ln, po, P, CT, err);
+      CHECK(ok == c.ok, "ln=%d cap=%u pairs=%u: %s", c.ln, c.cap, cp, err.c_str());
+      if (!ok) {
+        CHECK(err.find("too tall") != std::string::npos, "ln=%d cap=%u: %s", c.ln, c.cap, err.c_str());
+        continue;
+      }
+      CHECK(P.outer.A <= PASS_MAX_LOG_R && (1 << (P.outer.A + P.logW)) <= PASS_MAX_ELEMS, "ln=%d outer tile", c.ln);
+      if (P.has_mid)
+        CHECK(P.mid.A <= PASS_MAX_LOG_R && (1 << (P.mid.A + P.mid_logW)) <= PASS_MAX_ELEMS, "ln=%d mid tile", c.ln);
+    }
+  }
   printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
   return fails ? 1 : 0;
 }
