@@ -105,6 +105,41 @@ Pass make_pass(int ell0, int A, std::vector<e61>& tw61, std::vector<e31>& tw31, 
   return ps;
 }
 
+// Register twiddles of centre thread t (either field), as k_centre keeps
+// them: slot k (r - 1) + e - 1 holds tau^e of the butterfly the thread takes
+// in iteration k of the last step (lds_step's thread -> butterfly map with one
+// column and two blocks), slot CENTRE_STEP_SLOTS + k the squaring twiddle of
+// the pair it takes in iteration k (square_self's map for workgroup 0,
+// square_cross's otherwise). Unused slots stay 0.
+template <class E>
+void centre_register_twiddles(const Pass& cp, uint32_t g, const uint32_t bl[2], const std::vector<E>& tw,
+                              const std::vector<E>& sq, std::vector<E>& out) {
+  const int c = cp.A;
+  const Step& st = cp.steps[cp.nsteps - 1];
+  const int r = st.radix, lr = log2i(r), lam = st.ell - cp.ell0;
+  const int logd = c - lam - lr, logU = c - lr;
+  const int total = 2 << logU;
+  const int Cb = 1 << c, half = Cb >> 1;
+  const size_t base = out.size();
+  out.resize(base + (size_t)CENTRE_TW_SLOTS * CENTRE_ROLE_THREADS, E{});
+  auto at = [&](int slot, int t) -> E& { return out[base + (size_t)slot * CENTRE_ROLE_THREADS + t]; };
+  for (int t = 0; t < CENTRE_ROLE_THREADS; ++t) {
+    for (int k = 0; t + k * CENTRE_ROLE_THREADS < total; ++k) {
+      const int bt = t + k * CENTRE_ROLE_THREADS;
+      const int b = bt >> logU, u = bt & ((1 << logU) - 1), blk = u >> logd;
+      const uint32_t g0 = st.twoff + ((bl[b] << lam) + (uint32_t)blk) * (uint32_t)(r - 1);
+      for (int e = 1; e < r; ++e) at(k * (r - 1) + e - 1, t) = tw[g0 + e - 1];
+    }
+    for (int k = 0; k < CENTRE_SQ_SLOTS; ++k) {
+      const int o = t + k * CENTRE_ROLE_THREADS;
+      if (o >= Cb) break;  // both maps cover Cb pairs
+      const int ti = g == 0 ? o : ((o & 1) == 0 ? (o >> 1) : half + ((Cb - 1 - o) >> 1));
+      const int b = ti >= half ? 1 : 0;
+      at(CENTRE_STEP_SLOTS + k, t) = sq[((size_t)bl[b] << (c - 1)) + (size_t)(ti - b * half)];
+    }
+  }
+}
+
 }  // namespace
 
 bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, std::string& err) {
@@ -205,15 +240,24 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
   // step twiddles: M61 in the split form of mul_tw
   for (e61& w : T.tw61) w = split31(w);
   // per-workgroup centre records (contiguous, so a workgroup stages its
-  // twiddles with one batched copy)
+  // twiddles with one batched copy and loads its register twiddles with
+  // coalesced loads)
   if (!P.small) {
     const Pass& cp = P.centre;
     const int c = cp.A;
     const uint32_t ncentre = (uint32_t)((h >> c) / 2);
-    uint32_t nT = 0;
-    for (int si = 0; si < cp.nsteps; ++si)
-      nT += 2u * (1u << (cp.steps[si].ell - cp.ell0)) * (uint32_t)(cp.steps[si].radix - 1);
-    P.crec = nT + (1u << c);
+    uint32_t nE = 0;  // twiddles of the steps before the last (LDS)
+    for (int si = 0; si + 1 < cp.nsteps; ++si)
+      nE += 2u * (1u << (cp.steps[si].ell - cp.ell0)) * (uint32_t)(cp.steps[si].radix - 1);
+    const Step& last = cp.steps[cp.nsteps - 1];
+    const int lr = log2i(last.radix);
+    const int per_thread = ((2 << (c - lr)) + CENTRE_ROLE_THREADS - 1) / CENTRE_ROLE_THREADS;
+    if (last.radix < 4 || nE > (uint32_t)CENTRE_EARLY_TW || per_thread * (last.radix - 1) > CENTRE_STEP_SLOTS ||
+        (2 << (c - 1)) > CENTRE_SQ_SLOTS * CENTRE_ROLE_THREADS) {
+      err = "plan error: centre twiddles do not fit the workgroup's slots";
+      return false;
+    }
+    P.crec = nE + (uint32_t)(CENTRE_TW_SLOTS * CENTRE_ROLE_THREADS);
     T.ctw61.clear();
     T.ctw31.clear();
     T.ctw61.reserve((size_t)ncentre * P.crec);
@@ -221,7 +265,7 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
     for (uint32_t g = 0; g < ncentre; ++g) {
       uint32_t bl[2];
       centre_blocks(g, bl[0], bl[1]);
-      for (int si = 0; si < cp.nsteps; ++si) {
+      for (int si = 0; si + 1 < cp.nsteps; ++si) {
         const Step& st = cp.steps[si];
         const int lam = st.ell - cp.ell0;
         const uint32_t span = (1u << lam) * (uint32_t)(st.radix - 1);
@@ -231,11 +275,8 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
           T.ctw31.insert(T.ctw31.end(), T.tw31.begin() + g0, T.tw31.begin() + g0 + span);
         }
       }
-      for (int b = 0; b < 2; ++b) {
-        const size_t s0 = (size_t)bl[b] << (c - 1);
-        T.ctw61.insert(T.ctw61.end(), T.sq61.begin() + s0, T.sq61.begin() + s0 + (1u << (c - 1)));
-        T.ctw31.insert(T.ctw31.end(), T.sq31.begin() + s0, T.sq31.begin() + s0 + (1u << (c - 1)));
-      }
+      centre_register_twiddles(cp, g, bl, T.tw61, T.sq61, T.ctw61);
+      centre_register_twiddles(cp, g, bl, T.tw31, T.sq31, T.ctw31);
     }
   }
   return true;

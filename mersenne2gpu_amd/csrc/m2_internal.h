@@ -48,8 +48,10 @@ struct DevPlan {
   Pass centre{};
   uint32_t nseg = 0;  // number of carry segments (length of G)
   // Centre twiddles regrouped per centre workgroup: record g (crec entries)
-  // holds, for its blocks (B, B2), the step twiddles in the order the
-  // workgroup keeps them in LDS, then the squaring twiddles of B and B2.
+  // holds, for its blocks (B, B2), the twiddles of all but the last step in
+  // the order the workgroup keeps them in LDS, then CENTRE_TW_SLOTS slots of
+  // CENTRE_ROLE_THREADS entries: slot s, entry t is the twiddle thread t of
+  // either field keeps in register s (centre_register_twiddles).
   const e61* ctw61 = nullptr;
   const e31* ctw31 = nullptr;
   uint32_t crec = 0;

@@ -355,31 +355,36 @@ __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, c
 
 // Squaring of blocks that pair with themselves: block 0 (positions 0..Cb-1,
 // mirror formula, k = 0 special) at LDS offset 0, and, when nself = 2, block 1
-// (positions Cb..2Cb-1, k pairs with 3Cb-1-k) at LDS offset Cb. S61/S31 hold
-// the centre twiddles of the even positions: block 0's Cb/2, then block 1's.
-template <bool SPLIT>
-__device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int nself, const e61* S61,
-                                            const e31* S31) {
+// (positions Cb..2Cb-1, k pairs with 3Cb-1-k) at LDS offset Cb. The centre
+// twiddle of pair q (block 0's Cb/2 even positions, then block 1's) is
+// s61(k, q) / s31(k, q), k being the thread's iteration: from LDS, or from
+// the registers the centre keeps for iteration k (k < 4: a thread takes at
+// most four pairs, nself * Cb/2 <= 4 x 256).
+template <bool SPLIT, class S61, class S31>
+__device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int nself, S61 s61, S31 s31) {
   const Lanes<SPLIT> ln;
   const int Cb = 1 << logCb;
   const int half = Cb >> 1;
-  for (int q = ln.tid; q < nself * half; q += ln.n) {
+#pragma unroll 1
+  for (int k = 0; k < 4; ++k) {
+    const int q = ln.tid + k * ln.n;
+    if (q >= nself * half) break;
     const int blk = q >= half ? 1 : 0;
     const int j = q - blk * half;
-    const int k = 2 * j;
-    const int kidx = blk * Cb + k;
-    const int mkidx = blk ? Cb + (Cb - 1 - k) : (k ? (3 << (31 - __clz(k))) - k - 1 : 0);
-    if (blk == 0 && k == 0) {
+    const int kk = 2 * j;
+    const int kidx = blk * Cb + kk;
+    const int mkidx = blk ? Cb + (Cb - 1 - kk) : (kk ? (3 << (31 - __clz(kk))) - kk - 1 : 0);
+    if (blk == 0 && kk == 0) {
       if (ln.f61) {
         e61 a = L61[PADX(0)], a2 = a;
-        square_pair(a, a2, S61[q]);
+        square_pair(a, a2, s61(k, q));
         const e61 z1 = red(L61[PADX(1)]);
         L61[PADX(0)] = a;
         L61[PADX(1)] = sqr(add(z1, z1));
       }
       if (ln.f31) {
         e31 b = L31[PADX(0)], b2 = b;
-        square_pair(b, b2, S31[q]);
+        square_pair(b, b2, s31(k, q));
         const e31 y1 = L31[PADX(1)];
         L31[PADX(0)] = b;
         L31[PADX(1)] = sqr(add(y1, y1));
@@ -387,13 +392,13 @@ __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int n
     } else {
       if (ln.f61) {
         e61 a = L61[PADX(kidx)], c = L61[PADX(mkidx)];
-        square_pair(a, c, S61[q]);
+        square_pair(a, c, s61(k, q));
         L61[PADX(kidx)] = a;
         L61[PADX(mkidx)] = c;
       }
       if (ln.f31) {
         e31 b = L31[PADX(kidx)], d = L31[PADX(mkidx)];
-        square_pair(b, d, S31[q]);
+        square_pair(b, d, s31(k, q));
         L31[PADX(kidx)] = b;
         L31[PADX(mkidx)] = d;
       }
@@ -403,13 +408,16 @@ __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int n
 }
 
 // Squaring across two LDS blocks B (at 0) and B2 = 3*2^a - 1 - B (at Cb):
-// position B*Cb + o pairs with B2*Cb + (Cb - 1 - o). S61: centre twiddles of
-// block B's even positions (Cb/2) followed by block B2's (Cb/2).
-template <bool SPLIT>
-__device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, const e61* S61, const e31* S31) {
+// position B*Cb + o pairs with B2*Cb + (Cb - 1 - o). Twiddle index ti: block
+// B's even positions (Cb/2) followed by block B2's (Cb/2); s61/s31 as above.
+template <bool SPLIT, class S61, class S31>
+__device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, S61 s61, S31 s31) {
   const Lanes<SPLIT> ln;
   const int Cb = 1 << logCb;
-  for (int o = ln.tid; o < Cb; o += ln.n) {
+#pragma unroll 1
+  for (int k = 0; k < 4; ++k) {
+    const int o = ln.tid + k * ln.n;
+    if (o >= Cb) break;
     int kidx, mkidx, ti;
     if ((o & 1) == 0) {
       kidx = o;
@@ -422,18 +430,126 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, cons
     }
     if (ln.f61) {
       e61 a = L61[PADX(kidx)], c = L61[PADX(mkidx)];
-      square_pair(a, c, S61[ti]);
+      square_pair(a, c, s61(k, ti));
       L61[PADX(kidx)] = a;
       L61[PADX(mkidx)] = c;
     }
     if (ln.f31) {
       e31 b = L31[PADX(kidx)], d = L31[PADX(mkidx)];
-      square_pair(b, d, S31[ti]);
+      square_pair(b, d, s31(k, ti));
       L31[PADX(kidx)] = b;
       L31[PADX(mkidx)] = d;
     }
   }
   __syncthreads();
+}
+
+// tw[first + k] for a runtime k < N, by selects, so that the register array
+// is never indexed dynamically (which would move it to scratch memory).
+template <int N>
+__device__ __forceinline__ e61 pick_slot(const e61 (&tw)[CENTRE_TW_SLOTS], int first, int k) {
+  e61 r = tw[first];
+#pragma unroll
+  for (int j = 1; j < N; ++j) {
+    e61 c = tw[first + j];
+    // opaque copies: keeps the compiler from turning the selects back into a
+    // dynamically indexed load
+    asm volatile("" : "+v"(c.a), "+v"(c.b));
+    r.a = k == j ? c.a : r.a;
+    r.b = k == j ? c.b : r.b;
+  }
+  return r;
+}
+
+// The centre's last forward step (FWD) or first inverse step with this
+// thread's twiddles in registers: iteration k's butterfly uses slots
+// k (r - 1) .. k (r - 1) + r - 2 (M61 lanes: tw[s]; M31 lanes: the e31
+// packed in tw[s].a). One column, nb blocks, as lds_step with logW = 0.
+// Radix 8 (one butterfly per thread) or 4 (at most two).
+template <bool FWD, int RADIX>
+__device__ __forceinline__ void reg_step(e61* L61, e31* L31, int logR, int nb, int lam,
+                                         const e61 (&tw)[CENTRE_TW_SLOTS]) {
+  constexpr int LR = RADIX == 8 ? 3 : (RADIX == 4 ? 2 : 1);
+  constexpr int KMAX = CENTRE_STEP_SLOTS / (RADIX - 1);
+  const Lanes<true> ln;
+  const int logd = logR - lam - LR;
+  const int logU = logR - LR;
+  const int total = nb << logU;
+  static_assert(RADIX == 8 || RADIX == 4, "the centre's last step is radix 8 or 4");
+  // twiddle e (1 <= e < RADIX) of iteration k (radix 4: two iterations)
+  auto twk = [&](int k, int e) -> e61 {
+    if (KMAX == 1) return tw[e - 1];
+    const e61 lo = tw[e - 1];
+    e61 hi = tw[RADIX - 1 + e - 1];
+    asm volatile("" : "+v"(hi.a), "+v"(hi.b));
+    return e61{k ? hi.a : lo.a, k ? hi.b : lo.b};
+  };
+#pragma unroll 1
+  for (int k = 0; k < KMAX; ++k) {
+    const int t = ln.tid + k * ln.n;
+    if (t >= total) break;
+    const int b = t >> logU;
+    const int u = t & ((1 << logU) - 1);
+    const int off = u & ((1 << logd) - 1);
+    const int blk = u >> logd;
+    const int base = (b << logR) + (blk << (logR - lam)) + off;
+    const int stride = 1 << logd;
+    if (ln.f61) {
+      e61 v[RADIX];
+#pragma unroll
+      for (int e = 0; e < RADIX; ++e) v[e] = L61[PADX(base + e * stride)];
+      if (FWD) {
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) v[e] = mul_tw(v[e], twk(k, e));
+        if (RADIX == 8) dft8_fwd_lazy(v);
+        if (RADIX == 4) dft4_fwd_lazy(v);
+        if (RADIX == 2) dft2_fwd_lazy(v[0], v[1]);
+      } else {
+        if (RADIX == 8) dft8_inv_lazy(v);
+        if (RADIX == 4) dft4_inv_lazy(v);
+        if (RADIX == 2) dft2_inv_lazy(v[0], v[1]);
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) v[e] = mulconj_tw(v[e], twk(k, e));
+      }
+#pragma unroll
+      for (int e = 0; e < RADIX; ++e) L61[PADX(base + e * stride)] = v[e];
+    } else {
+      e31 x[RADIX];
+#pragma unroll
+      for (int e = 0; e < RADIX; ++e) x[e] = L31[PADX(base + e * stride)];
+      auto w31 = [&](int e) {
+        const e61 w = twk(k, e);
+        return e31{(uint32_t)w.a, (uint32_t)(w.a >> 32)};
+      };
+      if (FWD) {
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) x[e] = mul_tw(x[e], w31(e));
+        if (RADIX == 8) dft8_fwd(x);
+        if (RADIX == 4) dft4_fwd(x);
+        if (RADIX == 2) dft2(x[0], x[1]);
+      } else {
+        if (RADIX == 8) dft8_inv(x);
+        if (RADIX == 4) dft4_inv(x);
+        if (RADIX == 2) dft2(x[0], x[1]);
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) x[e] = mulconj_tw(x[e], w31(e));
+      }
+#pragma unroll
+      for (int e = 0; e < RADIX; ++e) L31[PADX(base + e * stride)] = x[e];
+    }
+  }
+  __syncthreads();
+}
+
+template <bool FWD>
+__device__ __forceinline__ void centre_last_step(e61* L61, e31* L31, const Pass& ps, int nb,
+                                                 const e61 (&tw)[CENTRE_TW_SLOTS]) {
+  const Step st = ps.steps[ps.nsteps - 1];
+  const int lam = st.ell - ps.ell0;
+  // the centre has at least 2 levels, so its last step is radix 8 or 4 (the
+  // host refuses other plans)
+  if (st.radix == 8) reg_step<FWD, 8>(L61, L31, ps.A, nb, lam, tw);
+  else reg_step<FWD, 4>(L61, L31, ps.A, nb, lam, tw);
 }
 
 // Copies count elements of both fields, global index src(i) -> LDS index
@@ -546,7 +662,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_small(const TestDesc* __restr
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_weight(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D);
     __syncthreads();
     lds_pass<true, true, true>(L61, L31, 0, 1, ps, T61, T31);
-    square_self<true>(L61, L31, logH, 1, S61, S31);
+    square_self<true>(L61, L31, logH, 1, [&](int, int q) { return S61[q]; }, [&](int, int q) { return S31[q]; });
     lds_pass<false, true, true>(L61, L31, 0, 1, ps, T61, T31);
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
       const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
@@ -823,15 +939,16 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
 // Centre pass: workgroup g squares the pair of contiguous blocks (B, B2) of
 // C = 2^logC elements that the real-data pairing links; workgroup 0 takes
 // blocks 0 and 1, which pair with themselves. Forward levels [ps.ell0, L),
-// square, inverse. There are h / (2C) workgroups.
+// square, inverse. There are h / (2C) workgroups. The twiddles of all but the
+// last step are staged in LDS; each thread loads the twiddles of its own
+// last-step butterflies and squaring pairs straight into registers, so the
+// workgroup needs 66 KB of LDS and two fit on a CU.
 __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
                                                 const e31* __restrict__ ctw31, uint32_t crec, Pass ps) {
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
-  __shared__ e61 T61[2 << CENTRE_LOG_C];
-  __shared__ e31 T31[2 << CENTRE_LOG_C];
-  __shared__ e61 S61[1 << CENTRE_LOG_C];
-  __shared__ e31 S31[1 << CENTRE_LOG_C];
+  __shared__ e61 T61[CENTRE_EARLY_TW];
+  __shared__ e31 T31[CENTRE_EARLY_TW];
   const TestDesc td = tests[blockIdx.y];
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
@@ -844,49 +961,42 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
     B = (1u << a) + (g - (1u << (a - 1)));
     B2 = 3u * (1u << a) - 1u - B;
   }
-  // The blocks' elements and the twiddles of all but the last forward step go
-  // to LDS first; the rest of this workgroup's twiddle record (the last
-  // step's twists, then the squaring twiddles: at most CPF per thread) is
-  // loaded into registers now and written to LDS after the other steps, so
-  // its loads overlap their arithmetic.
   gather2<4>(L61, L31, z61, z31, nb << logC,
              [&](int i) { return ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1)); },
              [](int i) { return PADX(i); });
-  const int nT = (int)crec - C;
-  int lo_last = 0;
-  for (int si = 0; si + 1 < ps.nsteps; ++si) lo_last += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
+  int nE = 0;
+  for (int si = 0; si + 1 < ps.nsteps; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
   const e61* r61 = ctw61 + (size_t)g * crec;
   const e31* r31 = ctw31 + (size_t)g * crec;
-  gather2<1>(T61, T31, r61, r31, lo_last, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
-  constexpr int CPF = 6;  // (2 (2^c - 1) + 2^c - lo_last) <= 3 2^10 = CPF * 512
-  uint64_t pa[CPF], pb[CPF];
-  uint32_t pc[CPF], pd[CPF];
+  gather2<1>(T61, T31, r61, r31, nE, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+  // register twiddles: slot s of thread t at record entry nE + s * 256 + t
+  // (coalesced); M31 lanes keep their e31 packed in .a
+  const Lanes<true> ln;
+  e61 tw[CENTRE_TW_SLOTS];
+  if (ln.f61) {
+    const M2_GLOBAL e61* src = gptr(r61 + nE) + ln.tid;
 #pragma unroll
-  for (int k = 0; k < CPF; ++k) {
-    const int i = min(lo_last + k * (int)blockDim.x + (int)threadIdx.x, (int)crec - 1);
-    pa[k] = r61[i].a;
-    pb[k] = r61[i].b;
-    pc[k] = r31[i].a;
-    pd[k] = r31[i].b;
-  }
-  __syncthreads();
-  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, 0, ps.nsteps - 1);
+    for (int k = 0; k < CENTRE_TW_SLOTS; ++k) tw[k] = gld(src + k * CENTRE_ROLE_THREADS);
+  } else {
+    const M2_GLOBAL e31* src = gptr(r31 + nE) + ln.tid;
 #pragma unroll
-  for (int k = 0; k < CPF; ++k) {
-    const int i = lo_last + k * (int)blockDim.x + (int)threadIdx.x;
-    if (i < nT) {
-      T61[i] = e61{pa[k], pb[k]};
-      T31[i] = e31{pc[k], pd[k]};
-    } else if (i < (int)crec) {
-      S61[i - nT] = e61{pa[k], pb[k]};
-      S31[i - nT] = e31{pc[k], pd[k]};
+    for (int k = 0; k < CENTRE_TW_SLOTS; ++k) {
+      const e31 w = gld(src + k * CENTRE_ROLE_THREADS);
+      tw[k] = e61{(uint64_t)w.a | ((uint64_t)w.b << 32), 0};
     }
   }
   __syncthreads();
-  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, ps.nsteps - 1, ps.nsteps);
-  if (g == 0) square_self<true>(L61, L31, logC, 2, S61, S31);
-  else square_cross<true>(L61, L31, logC, S61, S31);
-  lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31);
+  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, 0, ps.nsteps - 1);
+  centre_last_step<true>(L61, L31, ps, nb, tw);
+  auto s61 = [&](int k, int) { return pick_slot<CENTRE_SQ_SLOTS>(tw, CENTRE_STEP_SLOTS, k); };
+  auto s31 = [&](int k, int) {
+    const e61 w = pick_slot<CENTRE_SQ_SLOTS>(tw, CENTRE_STEP_SLOTS, k);
+    return e31{(uint32_t)w.a, (uint32_t)(w.a >> 32)};
+  };
+  if (g == 0) square_self<true>(L61, L31, logC, 2, s61, s31);
+  else square_cross<true>(L61, L31, logC, s61, s31);
+  centre_last_step<false>(L61, L31, ps, nb, tw);
+  lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31, 1, ps.nsteps);
   for (int i = threadIdx.x; i < (nb << logC); i += blockDim.x) {
     const uint32_t e = ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1));
     gst(z61 + e, L61[PADX(i)]);

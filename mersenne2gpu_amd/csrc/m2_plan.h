@@ -22,6 +22,15 @@ constexpr int SMALL_LOG_H = 11;     // h <= 2048 elements (48 KB of data in LDS)
 constexpr int CENTRE_LOG_C = 10;    // centre block: 1024 elements, two per workgroup
 constexpr int PASS_MAX_ELEMS = 2048;  // R * W elements per strided-pass workgroup
 constexpr int PASS_MAX_LOG_R = 9;     // strided passes: at most 2^9 rows (R - 1 twiddles)
+// Centre workgroups: 256 threads per field. Each thread keeps in registers
+// the twiddles of its butterflies in the last forward step (the first inverse
+// step uses them conjugated) and of its squaring pairs; only the other steps'
+// twiddles are staged in LDS.
+constexpr int CENTRE_ROLE_THREADS = 256;
+constexpr int CENTRE_STEP_SLOTS = 7;   // last step radix 8: 1 x 7, radix 4: 2 x 3
+constexpr int CENTRE_SQ_SLOTS = 4;     // 2^c <= 4 x 256 squaring pairs
+constexpr int CENTRE_TW_SLOTS = CENTRE_STEP_SLOTS + CENTRE_SQ_SLOTS;
+constexpr int CENTRE_EARLY_TW = 512;   // the other steps' twiddles, both blocks
 constexpr int SEG_ELEMS = 4;        // carry segment: 4 elements = 8 digits
 
 // One radix step inside a pass: global level `ell`, radix 2/4/8, and the
