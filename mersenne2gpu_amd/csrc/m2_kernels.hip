@@ -40,6 +40,9 @@ __device__ __forceinline__ int64_t to_i64(uint64_t x) { return (int64_t)x; }
 __device__ __forceinline__ int PADX(int i) { return i + (i >> 3); }
 constexpr int padded(int n) { return n + n / 8; }
 
+// Threads per workgroup of the multi-pass kernels: 256 per field.
+constexpr int MP_THREADS = 512;
+
 // Signed digit -> canonical field residues.
 __device__ __forceinline__ uint64_t digit61(int64_t d) { return d >= 0 ? (uint64_t)d : P61 - (uint64_t)(-d); }
 __device__ __forceinline__ uint32_t mod31_u64(uint64_t x) {
@@ -444,6 +447,128 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, S61 
   __syncthreads();
 }
 
+// A pass's first step with its inputs read straight from global memory
+// (LOAD), or its last step with its outputs written straight there (!LOAD):
+// the same butterflies, LDS layout and values as lds_step (SPLIT lanes), with
+// the thread's own elements at global index gidx(LDS index) and its twiddles
+// read from the global tables at twi(b, blk) + e - 1. A first step so needs
+// no barrier before it (each wave starts on its own loads as they arrive),
+// and a last step no LDS write-back and store loop after it.
+template <bool FWD, int RADIX, bool LOAD, class GI, class TI>
+__device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW, int nb, int lam,
+                                          M2_GLOBAL e61* z61, M2_GLOBAL e31* z31, GI gidx,
+                                          const e61* __restrict__ tw61, const e31* __restrict__ tw31, TI twi) {
+  constexpr int LR = RADIX == 8 ? 3 : (RADIX == 4 ? 2 : 1);
+  constexpr int KMAX = PASS_MAX_ELEMS / RADIX / (MP_THREADS / 2);  // butterflies per thread
+  const Lanes<true> ln;
+  const int logd = logR - lam - LR;
+  const int logU = logR - LR + logW;
+  const int total = nb << logU;
+  const int stride = 1 << (logd + logW);
+  int base[KMAX], tw[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int t = ln.tid + k * ln.n;
+    const int b = t >> logU;
+    const int u = t & ((1 << logU) - 1);
+    const int col = u & ((1 << logW) - 1);
+    const int g = u >> logW;
+    const int off = g & ((1 << logd) - 1);
+    const int blk = g >> logd;
+    base[k] = (b << (logR + logW)) + (((blk << (logR - lam)) + off) << logW) + col;
+    tw[k] = (int)twi(b, blk);
+  }
+  if (ln.f61) {
+    e61 v[KMAX][RADIX];
+    if (LOAD) {
+      // every load first, so their latencies overlap
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k)
+        if (ln.tid + k * ln.n < total) {
+#pragma unroll
+          for (int e = 0; e < RADIX; ++e) v[k][e] = gld(z61 + gidx(base[k] + e * stride));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      if (ln.tid + k * ln.n >= total) break;
+      if (!LOAD) {
+#pragma unroll
+        for (int e = 0; e < RADIX; ++e) v[k][e] = L61[PADX(base[k] + e * stride)];
+      }
+      const M2_GLOBAL e61* T = gptr(tw61 + tw[k]);
+      if (FWD) {
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) v[k][e] = mul_tw(v[k][e], gld(T + e - 1));
+        if (RADIX == 8) dft8_fwd_lazy(v[k]);
+        if (RADIX == 4) dft4_fwd_lazy(v[k]);
+        if (RADIX == 2) dft2_fwd_lazy(v[k][0], v[k][1]);
+      } else {
+        if (RADIX == 8) dft8_inv_lazy(v[k]);
+        if (RADIX == 4) dft4_inv_lazy(v[k]);
+        if (RADIX == 2) dft2_inv_lazy(v[k][0], v[k][1]);
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) v[k][e] = mulconj_tw(v[k][e], gld(T + e - 1));
+      }
+#pragma unroll
+      for (int e = 0; e < RADIX; ++e) {
+        if (LOAD) L61[PADX(base[k] + e * stride)] = v[k][e];
+        else gst(z61 + gidx(base[k] + e * stride), v[k][e]);
+      }
+    }
+  } else {
+    e31 x[KMAX][RADIX];
+    if (LOAD) {
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k)
+        if (ln.tid + k * ln.n < total) {
+#pragma unroll
+          for (int e = 0; e < RADIX; ++e) x[k][e] = gld(z31 + gidx(base[k] + e * stride));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      if (ln.tid + k * ln.n >= total) break;
+      if (!LOAD) {
+#pragma unroll
+        for (int e = 0; e < RADIX; ++e) x[k][e] = L31[PADX(base[k] + e * stride)];
+      }
+      const M2_GLOBAL e31* T = gptr(tw31 + tw[k]);
+      if (FWD) {
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) x[k][e] = mul_tw(x[k][e], gld(T + e - 1));
+        if (RADIX == 8) dft8_fwd(x[k]);
+        if (RADIX == 4) dft4_fwd(x[k]);
+        if (RADIX == 2) dft2(x[k][0], x[k][1]);
+      } else {
+        if (RADIX == 8) dft8_inv(x[k]);
+        if (RADIX == 4) dft4_inv(x[k]);
+        if (RADIX == 2) dft2(x[k][0], x[k][1]);
+#pragma unroll
+        for (int e = 1; e < RADIX; ++e) x[k][e] = mulconj_tw(x[k][e], gld(T + e - 1));
+      }
+#pragma unroll
+      for (int e = 0; e < RADIX; ++e) {
+        if (LOAD) L31[PADX(base[k] + e * stride)] = x[k][e];
+        else gst(z31 + gidx(base[k] + e * stride), x[k][e]);
+      }
+    }
+  }
+}
+
+// edge_step for step si of pass ps (radix dispatch). No barrier: the caller
+// adds it where the LDS results are read by other threads.
+template <bool FWD, bool LOAD, class GI, class TI>
+__device__ __forceinline__ void edge_pass_step(e61* L61, e31* L31, int logW, int nb, const Pass& ps, int si,
+                                               M2_GLOBAL e61* z61, M2_GLOBAL e31* z31, GI gidx,
+                                               const e61* tw61, const e31* tw31, TI twi) {
+  const Step st = ps.steps[si];
+  const int lam = st.ell - ps.ell0;
+  if (st.radix == 8) edge_step<FWD, 8, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
+  else if (st.radix == 4) edge_step<FWD, 4, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
+  else edge_step<FWD, 2, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
+}
+
 // tw[first + k] for a runtime k < N, by selects, so that the register array
 // is never indexed dynamically (which would move it to scratch memory).
 template <int N>
@@ -689,8 +814,6 @@ __device__ __forceinline__ uint32_t prev_strip_carry(uint32_t cg, uint32_t r, ui
   return (NS - 1) * R + (r ? r - 1 : R - 1);
 }
 
-// Threads per workgroup of the multi-pass kernels: 256 per field.
-constexpr int MP_THREADS = 512;
 
 // Workgroups are dealt to the 8 XCDs round-robin by their linear index, and
 // each XCD has its own L2. Neighbouring column strips share cache lines (a
@@ -854,13 +977,19 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
   if (threadIdx.x == 0) epoch = __hip_atomic_load(sync + cg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-  gather2<4>(L61, L31, z61, z31, R * W,
-             [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); },
-             [](int i) { return PADX(i); });
+  auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
+  // the inverse pass's first step (the forward's last) reads global memory
+  // directly, and the forward's last step writes it directly
+  const int ns = ps.nsteps;
+  const bool edges = ns >= 2;
+  const Step sl = ps.steps[ns - 1];
+  auto twl = [&](int, int blk) { return sl.twoff + (uint32_t)blk * (uint32_t)(sl.radix - 1); };
+  if (edges) edge_pass_step<false, true>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
+  else gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   // inverse outer pass, unweight + CRT, segment carries (as k_outer_inv)
-  lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31, edges ? 1 : 0);
   const int subs = W / SEG_ELEMS;
   for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
     const int r = it / subs, v = it - r * subs;
@@ -897,11 +1026,14 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
     seg_weight(L61, L31, idx0, 2u * e0, D);
   }
   __syncthreads();
-  lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31);
+  lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31, 0, edges ? ns - 1 : ns);
+  if (edges) {
+    edge_pass_step<true, false>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
+    return;
+  }
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
-    const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
-    gst(z61 + e, L61[PADX(i)]);
-    gst(z31 + e, L31[PADX(i)]);
+    gst(z61 + gidx(i), L61[PADX(i)]);
+    gst(z31 + gidx(i), L31[PADX(i)]);
   }
 }
 
@@ -961,13 +1093,22 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
     B = (1u << a) + (g - (1u << (a - 1)));
     B2 = 3u * (1u << a) - 1u - B;
   }
-  gather2<4>(L61, L31, z61, z31, nb << logC,
-             [&](int i) { return ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1)); },
-             [](int i) { return PADX(i); });
-  int nE = 0;
-  for (int si = 0; si + 1 < ps.nsteps; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
+  auto gidx = [&](int i) { return ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1)); };
   const e61* r61 = ctw61 + (size_t)g * crec;
   const e31* r31 = ctw31 + (size_t)g * crec;
+  // With two steps or more, the first forward step reads global memory
+  // directly and the last inverse step writes it directly (step 0's twiddles
+  // lead the record: block b's span, then block b+1's).
+  const int ns = ps.nsteps;
+  const bool edges = ns >= 2;
+  const Step s0 = ps.steps[0];
+  auto tw0 = [&](int b, int blk) {
+    return (uint32_t)(b * twiddle_span(s0, s0.ell - ps.ell0) + blk * (s0.radix - 1));
+  };
+  if (edges) edge_pass_step<true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
+  else gather2<4>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
+  int nE = 0;
+  for (int si = 0; si + 1 < ns; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
   gather2<1>(T61, T31, r61, r31, nE, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
   // register twiddles: slot s of thread t at record entry nE + s * 256 + t
   // (coalesced); M31 lanes keep their e31 packed in .a
@@ -986,7 +1127,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
     }
   }
   __syncthreads();
-  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, 0, ps.nsteps - 1);
+  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, edges ? 1 : 0, ns - 1);
   centre_last_step<true>(L61, L31, ps, nb, tw);
   auto s61 = [&](int k, int) { return pick_slot<CENTRE_SQ_SLOTS>(tw, CENTRE_STEP_SLOTS, k); };
   auto s31 = [&](int k, int) {
@@ -996,11 +1137,14 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
   if (g == 0) square_self<true>(L61, L31, logC, 2, s61, s31);
   else square_cross<true>(L61, L31, logC, s61, s31);
   centre_last_step<false>(L61, L31, ps, nb, tw);
-  lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31, 1, ps.nsteps);
+  lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31, 1, edges ? ns - 1 : ns);
+  if (edges) {
+    edge_pass_step<false, false>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
+    return;
+  }
   for (int i = threadIdx.x; i < (nb << logC); i += blockDim.x) {
-    const uint32_t e = ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1));
-    gst(z61 + e, L61[PADX(i)]);
-    gst(z31 + e, L31[PADX(i)]);
+    gst(z61 + gidx(i), L61[PADX(i)]);
+    gst(z31 + gidx(i), L31[PADX(i)]);
   }
 }
 
