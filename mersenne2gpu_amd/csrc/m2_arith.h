@@ -65,7 +65,7 @@ M2_HD uint64_t dot61(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
   return add61(mul61(a, b), mul61(c, d));
 #endif
 }
-// x * 2^s for 0 <= s < 61: rotate the 61-bit word left by s.
+// x * 2^s for 0 <= s <= 61: rotate the 61-bit word left by s (s = 61 gives x).
 M2_HD uint64_t rot61(uint64_t x, uint32_t s) { return ((x << s) & P61) | (x >> (61u - s)); }
 
 // ---- M31 ----
@@ -80,6 +80,7 @@ M2_HD uint32_t mul31(uint32_t x, uint32_t y) {
   const uint32_t r = (uint32_t(t) & P31) + uint32_t(t >> 31);
   return umin32(r, r - P31);
 }
+// x * 2^s for x <= P31, 0 <= s <= 31 (s = 31 gives x).
 M2_HD uint32_t rot31(uint32_t x, uint32_t s) {
   const uint64_t t = uint64_t(x) << s;
   return (uint32_t(t) & P31) | uint32_t(t >> 31);

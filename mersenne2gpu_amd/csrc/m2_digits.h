@@ -27,13 +27,15 @@ M2_HD void dwalk_next(const DigitGen& D, DWalk& w) {
   const uint32_t t = w.r + D.s;
   const uint32_t wrap = (t > D.nmask) ? 1u : 0u;
   w.r = t & D.nmask;
+  // a in [1, 122], b in [1, 62]: two unsigned-min steps (a - 61 wraps to a
+  // large value when a < 61) leave them in [0, 61) and [0, 31)
   uint32_t a = w.v61 + 61u - D.d61 + wrap;
-  a -= (a >= 61u) ? 61u : 0u;
-  a -= (a >= 61u) ? 61u : 0u;
+  a = umin32(a, a - 61u);
+  a = umin32(a, a - 61u);
   w.v61 = a;
   uint32_t b = w.v31 + 31u - D.d31 + wrap;
-  b -= (b >= 31u) ? 31u : 0u;
-  b -= (b >= 31u) ? 31u : 0u;
+  b = umin32(b, b - 31u);
+  b = umin32(b, b - 31u);
   w.v31 = b;
 }
 

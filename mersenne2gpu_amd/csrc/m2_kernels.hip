@@ -61,8 +61,8 @@ __device__ __forceinline__ i128 crt_digit(uint64_t s, uint32_t t, uint32_t w61, 
   a61 -= (a61 >= 61u) ? 61u : 0u;
   uint32_t a31 = w31 + D.k31;
   a31 -= (a31 >= 31u) ? 31u : 0u;
-  s = a61 ? rot61(s, 61u - a61) : s;
-  t = a31 ? rot31(t, 31u - a31) : t;
+  s = rot61(s, 61u - a61);  // a rotation by 61 (31) is the identity: no select for a = 0
+  t = rot31(t, 31u - a31);
   uint64_t u = sub61(s, t);
   u = add61(u, rot61(u, 31));
   // x = u (2^31 - 1) + t in [0, M), M = P61 P31 = P61 (2^31 - 1). Values at or

@@ -85,6 +85,17 @@ int main() {
     CHECK(rot30_lazy(y) < (1ull << 61) + (1ull << 33) && canon61(rot30_lazy(y)) == m61((u128)y << 30),
           "rot30_lazy");
   }
+  // rotations: by the full width they are the identity (the CRT relies on it)
+  for (int i = 0; i < N; ++i) {
+    const uint64_t x = pick(P61);
+    const uint32_t y = pick31();
+    for (uint32_t sh = 0; sh <= 61; ++sh)
+      CHECK(canon61(rot61(x, sh)) == m61((u128)x << sh) && rot61(x, sh) <= P61, "rot61 %u", sh);
+    CHECK(rot61(x, 61) == x && rot31(y, 31) == y, "full rotation");
+    for (uint32_t sh = 0; sh <= 31; ++sh)
+      CHECK(canon31(rot31(y, sh)) == (uint32_t)(((uint64_t)y << sh) % P31) && rot31(y, sh) <= P31, "rot31 %u", sh);
+    if (i > N / 16) break;
+  }
   // lazy butterflies against the reduced generic ones: forward inputs have
   // element 0 <= P61 + 7, the rest <= P61; inverse inputs <= P61
   for (int i = 0; i < N / 4; ++i) {
