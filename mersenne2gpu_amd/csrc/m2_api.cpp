@@ -164,7 +164,8 @@ int get_class(m2_ctx* ctx, int ln, const PlanOpts& po, Class** out) {
   c->P.sq61 = (const e61*)d;
   if ((rc = dev_alloc(c->allocs, c->bytes, &d, T.sq31.data(), T.sq31.size() * sizeof(e31)))) return bail(rc);
   c->P.sq31 = (const e31*)d;
-  c->fusable = !c->P.small && !c->P.has_mid && outer_strips(c->P) <= (uint32_t)ctx->prop.multiProcessorCount;
+  c->P.cus = (uint32_t)ctx->prop.multiProcessorCount;
+  c->fusable = !c->P.small && !c->P.has_mid && outer_strips(c->P) <= c->P.cus;
   if (!c->P.small) {
     if ((rc = dev_alloc(c->allocs, c->bytes, &d, T.ctw61.data(), T.ctw61.size() * sizeof(e61)))) return bail(rc);
     c->P.ctw61 = (const e61*)d;

@@ -40,6 +40,7 @@ struct DevPlan {
   uint32_t h = 0;
   int ln = 0, L = 0;
   bool small = false;
+  uint32_t cus = 0;  // compute units of the device the tables live on
   Pass outer{};  // small plan: the whole transform
   int logS = 0, logW = 0;
   bool has_mid = false;

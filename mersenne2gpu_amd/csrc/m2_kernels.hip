@@ -866,11 +866,21 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
     seg_weight(L61, L31, idx0, 2u * e0, D);
   }
   __syncthreads();
-  lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31);
+  // the last step (not level 0 when there are two or more) writes global
+  // memory directly
+  auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
+  const int ns = ps.nsteps;
+  const bool edges = ns >= 2;
+  lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31, 0, edges ? ns - 1 : ns);
+  if (edges) {
+    const Step sl = ps.steps[ns - 1];
+    edge_pass_step<true, false>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31,
+                                [&](int, int blk) { return sl.twoff + (uint32_t)blk * (uint32_t)(sl.radix - 1); });
+    return;
+  }
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
-    const uint32_t e = ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1));
-    gst(z61 + e, L61[PADX(i)]);
-    gst(z31 + e, L31[PADX(i)]);
+    gst(z61 + gidx(i), L61[PADX(i)]);
+    gst(z31 + gidx(i), L31[PADX(i)]);
   }
 }
 
@@ -1055,16 +1065,16 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
   const uint32_t jb = bx / NS, cg = bx - jb * NS;
   const uint32_t base = (jb << (ps.A + logS)) + (cg << logW);
   const uint32_t j0[1] = {jb};
-  gather2<4>(L61, L31, z61, z31, R * W,
-             [&](int i) { return base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1)); },
-             [](int i) { return PADX(i); });
+  // loads and stores go through LDS (direct global loads or stores at the
+  // pass edges measured slower here, where two workgroups share a CU)
+  auto gidx = [&](int i) { return base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1)); };
+  gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   lds_pass<FWD, true>(L61, L31, logW, 1, ps, T61, T31);
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
-    const uint32_t e = base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1));
-    gst(z61 + e, L61[PADX(i)]);
-    gst(z31 + e, L31[PADX(i)]);
+    gst(z61 + gidx(i), L61[PADX(i)]);
+    gst(z31 + gidx(i), L31[PADX(i)]);
   }
 }
 
@@ -1076,7 +1086,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
 // last-step butterflies and squaring pairs straight into registers, so the
 // workgroup needs 66 KB of LDS and two fit on a CU.
 __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
-                                                const e31* __restrict__ ctw31, uint32_t crec, Pass ps) {
+                                                const e31* __restrict__ ctw31, uint32_t crec, Pass ps,
+                                                int direct_in) {
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
   __shared__ e61 T61[CENTRE_EARLY_TW];
@@ -1096,16 +1107,19 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
   auto gidx = [&](int i) { return ((i < C ? B : B2) << logC) + (uint32_t)(i & (C - 1)); };
   const e61* r61 = ctw61 + (size_t)g * crec;
   const e31* r31 = ctw31 + (size_t)g * crec;
-  // With two steps or more, the first forward step reads global memory
-  // directly and the last inverse step writes it directly (step 0's twiddles
-  // lead the record: block b's span, then block b+1's).
+  // With two steps or more, the last inverse step writes global memory
+  // directly, and, with direct_in (a grid of at most one workgroup per CU,
+  // where no other workgroup's arithmetic overlaps the loads), the first
+  // forward step reads it directly too; step 0's twiddles lead the record
+  // (block b's span, then block b+1's).
   const int ns = ps.nsteps;
   const bool edges = ns >= 2;
+  const bool edge_in = edges && direct_in;
   const Step s0 = ps.steps[0];
   auto tw0 = [&](int b, int blk) {
     return (uint32_t)(b * twiddle_span(s0, s0.ell - ps.ell0) + blk * (s0.radix - 1));
   };
-  if (edges) edge_pass_step<true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
+  if (edge_in) edge_pass_step<true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
   else gather2<4>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
   int nE = 0;
   for (int si = 0; si + 1 < ns; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
@@ -1127,7 +1141,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
     }
   }
   __syncthreads();
-  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, edges ? 1 : 0, ns - 1);
+  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, edge_in ? 1 : 0, ns - 1);
   centre_last_step<true>(L61, L31, ps, nb, tw);
   auto s61 = [&](int k, int) { return pick_slot<CENTRE_SQ_SLOTS>(tw, CENTRE_STEP_SLOTS, k); };
   auto s31 = [&](int k, int) {
@@ -1177,7 +1191,9 @@ hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& l
                                P.mid_logW)) != hipSuccess)
     return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
-  if ((e = launch("k_centre", k_centre, dim3(ncentre, nt), tests, P.ctw61, P.ctw31, P.crec, P.centre)) != hipSuccess)
+  const int direct_in = ncentre * nt <= P.cus ? 1 : 0;
+  if ((e = launch("k_centre", k_centre, dim3(ncentre, nt), tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) !=
+      hipSuccess)
     return e;
   if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid, nt), tests, P.tw61, P.tw31, P.mid, P.mid_logS,
                                P.mid_logW)) != hipSuccess)
@@ -1193,18 +1209,20 @@ hipError_t run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& la
   hipError_t e;
   const uint32_t ns = outer_strips(P);
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
+  const int direct_in = ncentre <= P.cus ? 1 : 0;
   if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
       hipSuccess)
     return e;
   for (uint64_t i = 0; i + 1 < k; ++i) {
-    if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre)) !=
+    if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) !=
         hipSuccess)
       return e;
     if ((e = launch("k_outer_fused", k_outer_fused, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW,
                     g_fused_spin_limit)) != hipSuccess)
       return e;
   }
-  if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre)) != hipSuccess)
+  if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) !=
+      hipSuccess)
     return e;
   return launch("k_outer_inv", k_outer_inv, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
 }

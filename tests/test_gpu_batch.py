@@ -5,6 +5,7 @@ single-test parity suite already checks the engine against one exponent at a
 time. Bit-exact Res64 and residue hashes, no tolerance.
 """
 import hashlib
+import re
 
 import pytest
 
@@ -199,6 +200,17 @@ def test_fused_and_separate_outer_passes_agree(gpu_ctx):
                 a.run(k - a.iteration)
                 b.run(k - b.iteration)
                 assert a.res64() == b.res64() == c["checkpoints"][str(k)]["res64"], (c["p"], k)
+    # one-step outer passes (n = 2^13, 2^14 with the largest centre: 2 and 3
+    # outer levels), whose fused pass stages through LDS instead of reading
+    # and writing global memory directly at the pass edges
+    one_step = {c["ln"]: c for c in _sweep_cases() if c["ln"] in (13, 14) and c["mode"] == "prp"}
+    for ln, c in sorted(one_step.items()):
+        with m2.Test(gpu_ctx, c["p"], "prp", local_stage_cap=2048) as a, \
+                m2.Test(gpu_ctx, c["p"], "prp", local_stage_cap=2048, no_fuse=1) as b:
+            assert re.search(r"levels 0\.\.%d radix \d\n" % (ln - 12), a.plan()), a.plan()
+            a.run(50)
+            b.run(50)
+            assert a.res64() == b.res64() == c["checkpoints"]["50"]["res64"], (c["p"], 50)
     c2 = load_fixture("c2.json")["cases"][0]
     with m2.Test(gpu_ctx, c2["p"], "prp", no_fuse=1) as b:
         b.run(100)
