@@ -18,9 +18,12 @@
 // the bits below it exceed the bits of g (8 digits of >= 12 bits each).
 // Convolution terms then stay below n * 2^(2q+2) <= 2^91 < M61*M31.
 //
-// Twiddles: every workgroup first copies the twiddle blocks of all its radix
-// steps (and of the centre's squaring) into LDS with coalesced loads, so no
-// radix step waits on global memory.
+// Twiddles: a workgroup copies the twiddle blocks of its interior radix steps
+// into LDS with coalesced loads at the start. The centre keeps the twiddles of
+// its last forward step, its squaring and its first inverse step in
+// registers (one coalesced record slot per thread); a step that reads or
+// writes global memory directly at a pass edge (edge_step) loads its own
+// twiddles from the global tables.
 #include <hip/hip_ext.h>
 
 #include "m2_arith.h"
@@ -30,7 +33,6 @@
 namespace m2 {
 
 typedef __int128 i128;
-
 
 __device__ __forceinline__ int64_t to_i64(uint64_t x) { return (int64_t)x; }
 
