@@ -198,7 +198,10 @@ int upload_digits(TestObj* t, const int64_t* digits, hipStream_t st) {
   return M2_OK;
 }
 
-int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, const PlanOpts& po, hipStream_t st, TestObj** out) {
+// fused: the test may run on the fused outer pass (single tests), which needs
+// the per-strip launch counters and the tagged carry words.
+int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, const PlanOpts& po, hipStream_t st, bool fused,
+                    TestObj** out) {
   if (mode != M2_MODE_LL && mode != M2_MODE_PRP) return fail(M2_E_INVAL, "mode must be LL (0) or PRP (1)");
   if (p < 3 || p > 0xFFFFFFFFull) return fail(M2_E_RANGE, "exponent out of range (3 <= p < 2^32)");
   const int ln = transform_log2(p);
@@ -230,7 +233,7 @@ int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, const PlanOpts& po, hipSt
     if ((rc = dev_alloc(t->allocs, t->bytes, &d, nullptr, P.nseg * sizeof(Carry128)))) return bail(rc);
     t->desc.G = (Carry128*)d;
   }
-  if (cls->fusable) {
+  if (fused && cls->fusable) {
     const size_t nsync = (outer_strips(P) + 1) * sizeof(uint32_t);
     if ((rc = dev_alloc(t->allocs, t->bytes, &d, nullptr, nsync))) return bail(rc);
     t->desc.sync = (uint32_t*)d;
@@ -578,7 +581,7 @@ int m2_test_create(m2_ctx* ctx, uint64_t p, int mode, const m2_opts* opts, m2_te
   int rc = get_class(ctx, ln, po, &cls);
   if (rc) return rc;
   if ((rc = t->group.create(cls))) return rc;
-  if ((rc = test_obj_create(ctx, p, mode, po, t->group.stream, &t->obj))) {
+  if ((rc = test_obj_create(ctx, p, mode, po, t->group.stream, true, &t->obj))) {
     t->group.destroy();
     return rc;
   }
@@ -938,7 +941,7 @@ int time_candidate(m2_ctx* ctx, uint64_t p, uint32_t ntests, const m2_opts& o, d
   };
   for (uint32_t i = 0; i < ntests; ++i) {
     TestObj* t = nullptr;
-    if ((rc = test_obj_create(ctx, p, M2_MODE_PRP, po, g.stream, &t))) return cleanup(rc);
+    if ((rc = test_obj_create(ctx, p, M2_MODE_PRP, po, g.stream, g.fused, &t))) return cleanup(rc);
     t->target = ~0ull;
     ts.push_back(t);
   }
@@ -1125,7 +1128,7 @@ int m2_batch_create(m2_ctx* ctx, size_t count, const uint64_t* ps, const int* mo
       it = b->groups.emplace(cls, g).first;
     }
     TestObj* t = nullptr;
-    if ((rc = test_obj_create(ctx, ps[i], modes[i], plan_opts(&o), it->second.stream, &t))) return bail(rc);
+    if ((rc = test_obj_create(ctx, ps[i], modes[i], plan_opts(&o), it->second.stream, false, &t))) return bail(rc);
     if (iters && iters[i]) t->target = iters[i];
     b->tests.push_back(t);
   }
@@ -1303,7 +1306,7 @@ int server_admit(m2_server* s, bool* progressed) {
       it = s->lanes.emplace(cls, lane).first;
     }
     TestObj* t = nullptr;
-    if ((rc = test_obj_create(s->ctx, rq.p, rq.mode, plan_opts(&o), s->setup, &t))) return rc;
+    if ((rc = test_obj_create(s->ctx, rq.p, rq.mode, plan_opts(&o), s->setup, false, &t))) return rc;
     if (rq.iters) t->target = std::min<uint64_t>(t->target, rq.iters);
     it->second.members.push_back({rq.id, t, fp});
     s->in_use += fp;
