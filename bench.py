@@ -13,6 +13,9 @@ Prints one JSON line. Extra fields:
                 duration (HIP events on the test's stream) against 8 TB/s;
                 traffic = measured HBM bytes per launch from the committed
                 rocprofv3 counter summary, when present;
+  int_roofline  the integer co-bound: algorithmic base products per iteration
+                (49.5 h per field) per second, and the dominant kernel's VALU
+                busy fraction per SIMD from the committed counter summary;
   cpu_baseline  the single-threaded CPU restatement of the reference's
                 algorithm (oracle/) timed on a bounded sample on this host.
 """
@@ -53,6 +56,19 @@ def load_traffic(kernel: str, workload: str):
         with open(path) as f:
             data = json.load(f)
         return data.get(workload, {}).get(kernel)
+    except Exception:
+        return None
+
+
+def load_valu(kernel: str, workload: str):
+    """VALU busy fraction per SIMD of a kernel, from the committed rocprofv3
+    shader-counter summary (profiles/valu.json), when present."""
+    path = os.path.join(ROOT, "profiles", "valu.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload, {}).get(kernel)
     except Exception:
         return None
 
@@ -182,6 +198,11 @@ def main() -> int:
     achieved = bytes_dom / (per_kernel_us[dom] * 1e-6) / 1e9
     traffic = load_traffic(dom, wl["traffic_key"])
     whole_gbps = 48.0 * n / (us_per_iter * 1e-6) / 1e9
+    # secondary (integer) roofline: the transform's algorithmic base products,
+    # 49.5 h per field per iteration (forward and inverse 7.25 h complex
+    # products each at radix-4 counting, 3 base products apiece, plus the
+    # centre's 6 h), against the measured time
+    modmuls = 2 * 49.5 * h
     t.close()
     ctx.close()
 
@@ -216,6 +237,11 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "bytes_per_launch": bytes_dom, "us_per_launch": round(per_kernel_us[dom], 3)},
+        "int_roofline": {"modmuls_per_iter": int(modmuls),
+                         "achieved_gmodmul_s": round(modmuls / (us_per_iter * 1e-6) / 1e9, 1),
+                         "kernel": dom, "valu_busy": load_valu(dom, wl["traffic_key"]),
+                         "valu_busy_source": "profiles/valu.json (rocprofv3 SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES, "
+                                             "two waves per SIMD)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
