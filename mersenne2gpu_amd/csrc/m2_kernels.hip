@@ -121,10 +121,21 @@ __device__ __forceinline__ void store_carry(CP G, uint32_t i, i128 v) {
   G[i].hi = (int64_t)(v >> 64);
 }
 
-// Segment helpers: 4 elements (8 digits) in LDS at idx0, first digit j0.
-// Unweight + CRT + zero-carry-in normalisation; returns the carry-out.
-__device__ __forceinline__ i128 seg_normalise(e61* L61, const e31* L31, int idx0, uint32_t j0, const DigitGen& D,
-                                              i128 carry) {
+// Carry segments: 4 elements (8 digits), first digit j0. In every kernel
+// that carries, each thread owns at most one segment (a workgroup holds at
+// most 2048 elements, 512 segments, with 512 threads) and keeps its 8 digits
+// in registers from the normalisation to the weighting, so the digits do not
+// go through LDS between the carry steps.
+static_assert(PASS_MAX_ELEMS / SEG_ELEMS <= MP_THREADS, "one carry segment per thread");
+static_assert((1 << SMALL_LOG_H) / SEG_ELEMS <= MP_THREADS, "one carry segment per thread");
+struct SegDigits {
+  int64_t d[2 * SEG_ELEMS];
+};
+
+// Unweight + CRT + zero-carry-in normalisation of the segment's spectrum at
+// LDS index idx0 into registers; returns the carry-out.
+__device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, int idx0, uint32_t j0,
+                                              const DigitGen& D, i128 carry, SegDigits& s) {
   DWalk w = dwalk_at(D, j0);
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
@@ -136,44 +147,34 @@ __device__ __forceinline__ i128 seg_normalise(e61* L61, const e31* L31, int idx0
     const uint32_t wb = dwalk_width(D, w);
     i128 xb = crt_digit(v.b, y.b, dwalk_w61(w), dwalk_w31(w), D);
     dwalk_next(D, w);
-    const int64_t da = norm_digit(xa + carry, wa, carry);
-    const int64_t db = norm_digit(xb + carry, wb, carry);
-    L61[PADX(idx0 + i)] = {(uint64_t)da, (uint64_t)db};
+    s.d[2 * i] = norm_digit(xa + carry, wa, carry);
+    s.d[2 * i + 1] = norm_digit(xb + carry, wb, carry);
   }
   return carry;
 }
 
 // Add carry c at the segment's first digit; the top digit absorbs the rest.
-__device__ __forceinline__ void seg_ripple(e61* L61, int idx0, uint32_t j0, const DigitGen& D, i128 c) {
+__device__ __forceinline__ void seg_ripple(SegDigits& s, uint32_t j0, const DigitGen& D, i128 c) {
   if (c == 0) return;
   DWalk w = dwalk_at(D, j0);
-  int64_t d[2 * SEG_ELEMS];
-#pragma unroll
-  for (int i = 0; i < SEG_ELEMS; ++i) {
-    const e61 v = L61[PADX(idx0 + i)];
-    d[2 * i] = to_i64(v.a);
-    d[2 * i + 1] = to_i64(v.b);
-  }
 #pragma unroll
   for (int i = 0; i < 2 * SEG_ELEMS - 1; ++i) {
     if (c != 0) {
-      i128 acc = (i128)d[i] + c;
-      d[i] = norm_digit(acc, dwalk_width(D, w), c);
+      i128 acc = (i128)s.d[i] + c;
+      s.d[i] = norm_digit(acc, dwalk_width(D, w), c);
     }
     dwalk_next(D, w);
   }
-  d[2 * SEG_ELEMS - 1] += (int64_t)c;
-#pragma unroll
-  for (int i = 0; i < SEG_ELEMS; ++i) L61[PADX(idx0 + i)] = {(uint64_t)d[2 * i], (uint64_t)d[2 * i + 1]};
+  s.d[2 * SEG_ELEMS - 1] += (int64_t)c;
 }
 
-// Digits of a segment -> weighted field elements in place.
-__device__ __forceinline__ void seg_weight(e61* L61, e31* L31, int idx0, uint32_t j0, const DigitGen& D) {
+// The segment's digits -> weighted field elements at LDS index idx0.
+__device__ __forceinline__ void seg_weight(const SegDigits& s, e61* L61, e31* L31, int idx0, uint32_t j0,
+                                           const DigitGen& D) {
   DWalk w = dwalk_at(D, j0);
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
-    const e61 v = L61[PADX(idx0 + i)];
-    const int64_t da = to_i64(v.a), db = to_i64(v.b);
+    const int64_t da = s.d[2 * i], db = s.d[2 * i + 1];
     const uint32_t a61 = dwalk_w61(w), a31 = dwalk_w31(w);
     dwalk_next(D, w);
     const uint32_t b61 = dwalk_w61(w), b31 = dwalk_w31(w);
@@ -181,6 +182,25 @@ __device__ __forceinline__ void seg_weight(e61* L61, e31* L31, int idx0, uint32_
     L61[PADX(idx0 + i)] = {rot61(digit61(da), a61), rot61(digit61(db), b61)};
     L31[PADX(idx0 + i)] = {rot31(digit31(da), a31), rot31(digit31(db), b31)};
   }
+}
+
+// Segment digits to / from global memory (digit pairs as int64, 64 contiguous
+// bytes per segment).
+__device__ __forceinline__ void seg_store(M2_GLOBAL e61* dst, const SegDigits& s) {
+#pragma unroll
+  for (int i = 0; i < SEG_ELEMS; ++i) gst(dst + i, e61{(uint64_t)s.d[2 * i], (uint64_t)s.d[2 * i + 1]});
+}
+__device__ __forceinline__ void seg_load(const M2_GLOBAL e61* src, SegDigits& s) {
+#pragma unroll
+  for (int i = 0; i < SEG_ELEMS; ++i) {
+    const e61 v = gld(src + i);
+    s.d[2 * i] = to_i64(v.a);
+    s.d[2 * i + 1] = to_i64(v.b);
+  }
+}
+__device__ __forceinline__ i128 carry_at(const M2_GLOBAL Carry128* G, uint32_t i) {
+  const Carry128 c = gld(G + i);
+  return (((i128)c.hi) << 64) | (i128)c.lo;
 }
 
 // ---------------------------------------------------------------------------
@@ -783,24 +803,28 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_small(const TestDesc* __restr
   const uint32_t j0[1] = {0};
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   stage_range(S61, S31, sq61, sq31, 0, h >> 1);
-  for (int i = threadIdx.x; i < h; i += blockDim.x) L61[PADX(i)] = gld(z61 + i);
-  __syncthreads();
+  // thread s owns carry segment s (digits 8s..8s+7) for the whole launch:
+  // its digits stay in registers between iterations
+  const int s = threadIdx.x;
+  const bool mine = s < nseg;
+  const int idx0 = s * SEG_ELEMS;
+  const uint32_t j = 2u * (uint32_t)idx0;
+  SegDigits sd;
+  if (mine) seg_load(z61 + idx0, sd);
   for (uint32_t it = 0; it < td.iters; ++it) {
-    for (int s = threadIdx.x; s < nseg; s += blockDim.x) seg_weight(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D);
+    if (mine) seg_weight(sd, L61, L31, idx0, j, D);
     __syncthreads();
     lds_pass<true, true, true>(L61, L31, 0, 1, ps, T61, T31);
     square_self<true>(L61, L31, logH, 1, [&](int, int q) { return S61[q]; }, [&](int, int q) { return S31[q]; });
     lds_pass<false, true, true>(L61, L31, 0, 1, ps, T61, T31);
-    for (int s = threadIdx.x; s < nseg; s += blockDim.x) {
+    if (mine) {
       const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
-      store_carry(gs, s, seg_normalise(L61, L31, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D, c0));
+      store_carry(gs, s, seg_normalise(L61, L31, idx0, j, D, c0, sd));
     }
     __syncthreads();
-    for (int s = threadIdx.x; s < nseg; s += blockDim.x)
-      seg_ripple(L61, s * SEG_ELEMS, 2u * s * SEG_ELEMS, D, load_carry(gs, (s + nseg - 1) % nseg));
-    __syncthreads();
+    if (mine) seg_ripple(sd, j, D, load_carry(gs, (s + nseg - 1) % nseg));
   }
-  for (int i = threadIdx.x; i < h; i += blockDim.x) gst(z61 + i, L61[PADX(i)]);
+  if (mine) seg_store(z61 + idx0, sd);
 }
 
 // ---------------------------------------------------------------------------
@@ -840,7 +864,6 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
-  __shared__ Carry128 gcin[PASS_MAX_ELEMS / SEG_ELEMS];
   const TestDesc td = tests[blockIdx.y];
   const M2_GLOBAL e61* __restrict__ zd = gptr(td.zd);
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
@@ -851,21 +874,22 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
   const uint32_t col0 = cg << logW;
   const uint32_t j0[1] = {0};
-  // digits of the strip are contiguous (loads batched), then the carry
-  // entering each strip row
-  const M2_GLOBAL e61* src = zd + (size_t)cg * (R * W);
-  gather1<4>(L61, src, R * W, [](int i) { return (uint32_t)i; }, [](int i) { return PADX(i); });
-  for (int rr = threadIdx.x; rr < R; rr += blockDim.x) gcin[rr] = gld(G + prev_strip_carry(cg, rr, R, NS));
+  // each thread takes one carry segment: its digits straight from the
+  // strip-major digit buffer (64 contiguous bytes), the carry entering its
+  // strip row if it is the row's first segment, then the weights into LDS
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
-  __syncthreads();
-  // prologue: carry-in of each strip-row, then weight
-  const int subs = W / SEG_ELEMS;
-  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
-    const int r = it / subs, v = it - r * subs;
-    const int idx0 = r * W + v * SEG_ELEMS;
-    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
-    if (v == 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gcin, (uint32_t)r));
-    seg_weight(L61, L31, idx0, 2u * e0, D);
+  {
+    const int subs = W / SEG_ELEMS;
+    const int it = threadIdx.x;
+    if (it < R * subs) {
+      const int r = it / subs, v = it - r * subs;
+      const int idx0 = r * W + v * SEG_ELEMS;
+      const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
+      SegDigits sd;
+      seg_load(zd + (size_t)cg * (R * W) + idx0, sd);
+      if (v == 0) seg_ripple(sd, 2u * e0, D, carry_at(G, prev_strip_carry(cg, (uint32_t)r, R, NS)));
+      seg_weight(sd, L61, L31, idx0, 2u * e0, D);
+    }
   }
   __syncthreads();
   // the last step (not level 0 when there are two or more) writes global
@@ -910,21 +934,28 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
   lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31);
+  // thread `it` owns carry segment it: normalise into registers, take the
+  // carry of the segment before it in the strip row, write the row's last
+  // carry to G; the digits go back through LDS so that the store to the
+  // strip-major digit buffer is one 16-byte element per lane (a segment per
+  // lane, 64 bytes apart, measured slower)
   const int subs = W / SEG_ELEMS;
-  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
-    const int r = it / subs, v = it - r * subs;
-    const int idx0 = r * W + v * SEG_ELEMS;
-    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
+  const int it = threadIdx.x;
+  const bool mine = it < R * subs;
+  const int r = it / subs, v = it - r * subs;
+  const int idx0 = r * W + v * SEG_ELEMS;
+  const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
+  SegDigits sd;
+  if (mine) {
     const i128 c0 = (mode == 0 && e0 == 0) ? (i128)(-2) : (i128)0;
-    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0));
+    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0, sd));
   }
   __syncthreads();
-  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
-    const int r = it / subs, v = it - r * subs;
-    const int idx0 = r * W + v * SEG_ELEMS;
-    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
-    if (v > 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gs, it - 1));
+  if (mine) {
+    if (v > 0) seg_ripple(sd, 2u * e0, D, load_carry(gs, it - 1));
     if (v == subs - 1) gst(G + cg * (uint32_t)R + (uint32_t)r, gs[it]);
+#pragma unroll
+    for (int i = 0; i < SEG_ELEMS; ++i) L61[PADX(idx0 + i)] = e61{(uint64_t)sd.d[2 * i], (uint64_t)sd.d[2 * i + 1]};
   }
   __syncthreads();
   M2_GLOBAL e61* dst = zd + (size_t)cg * (R * W);
@@ -1000,23 +1031,20 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   else gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
-  // inverse outer pass, unweight + CRT, segment carries (as k_outer_inv)
+  // inverse outer pass, unweight + CRT, segment carries (as k_outer_inv);
+  // thread `it` owns carry segment it and keeps its digits in registers
+  // until the weighting
   lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31, edges ? 1 : 0);
   const int subs = W / SEG_ELEMS;
-  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
-    const int r = it / subs, v = it - r * subs;
-    const int idx0 = r * W + v * SEG_ELEMS;
-    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
+  const int it = threadIdx.x;
+  const bool mine = it < R * subs;
+  const int r = it / subs, v = it - r * subs;
+  const int idx0 = r * W + v * SEG_ELEMS;
+  const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
+  SegDigits sd;
+  if (mine) {
     const i128 c0 = (mode == 0 && e0 == 0) ? (i128)(-2) : (i128)0;
-    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0));
-  }
-  __syncthreads();
-  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
-    const int r = it / subs, v = it - r * subs;
-    const int idx0 = r * W + v * SEG_ELEMS;
-    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
-    if (v > 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gs, it - 1));
-    if (v == subs - 1) carry_put(cf, cg * (uint32_t)R + (uint32_t)r, load_carry(gs, it), epoch & 0xFFu);
+    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0, sd));
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1025,17 +1053,15 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
     // exercise the report path)
     if (spin_limit == 0) __hip_atomic_store(sync + NS, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  // carry entering each strip row, then weight and the forward outer pass
-  // (as k_outer_fwd; gs is reused for the incoming carries)
-  for (int rr = threadIdx.x; rr < R; rr += blockDim.x)
-    store_carry(gs, rr, carry_take(cf, prev_strip_carry(cg, rr, R, NS), epoch & 0xFFu, spin_limit, sync + NS));
-  __syncthreads();
-  for (int it = threadIdx.x; it < R * subs; it += blockDim.x) {
-    const int r = it / subs, v = it - r * subs;
-    const int idx0 = r * W + v * SEG_ELEMS;
-    const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
-    if (v == 0) seg_ripple(L61, idx0, 2u * e0, D, load_carry(gs, (uint32_t)r));
-    seg_weight(L61, L31, idx0, 2u * e0, D);
+  // the carry of the segment before this one in its strip row; the row's
+  // last carry is published; the row's first segment then takes the carry
+  // the neighbour strip published; weight and the forward outer pass follow
+  if (mine) {
+    if (v > 0) seg_ripple(sd, 2u * e0, D, load_carry(gs, it - 1));
+    if (v == subs - 1) carry_put(cf, cg * (uint32_t)R + (uint32_t)r, load_carry(gs, it), epoch & 0xFFu);
+    if (v == 0)
+      seg_ripple(sd, 2u * e0, D, carry_take(cf, prev_strip_carry(cg, r, R, NS), epoch & 0xFFu, spin_limit, sync + NS));
+    seg_weight(sd, L61, L31, idx0, 2u * e0, D);
   }
   __syncthreads();
   lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31, 0, edges ? ns - 1 : ns);
