@@ -240,8 +240,8 @@ def main() -> int:
         "int_roofline": {"modmuls_per_iter": int(modmuls),
                          "achieved_gmodmul_s": round(modmuls / (us_per_iter * 1e-6) / 1e9, 1),
                          "kernel": dom, "valu_busy": load_valu(dom, wl["traffic_key"]),
-                         "valu_busy_source": "profiles/valu.json (rocprofv3 SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES, "
-                                             "two waves per SIMD)"},
+                         "valu_busy_source": "profiles/valu.json: rocprofv3 SQ counters, VALU instructions "
+                                             "per SIMD per 4-cycle issue slot"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
