@@ -39,7 +39,29 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(M2_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-constexpr uint32_t SMALL_CHUNK = 4096;  // max iterations per test in one single-workgroup launch
+// HIP events destroyed on every return path of the function that owns them.
+struct Events {
+  std::vector<hipEvent_t> ev;
+  Events() = default;
+  Events(const Events&) = delete;
+  Events& operator=(const Events&) = delete;
+  ~Events() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  hipError_t create(size_t n) {
+    ev.reserve(ev.size() + n);
+    for (size_t i = 0; i < n; ++i) {
+      hipEvent_t e = nullptr;
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      ev.push_back(e);
+    }
+    return hipSuccess;
+  }
+  hipEvent_t operator[](size_t i) const { return ev[i]; }
+};
+
+constexpr uint32_t SMALL_CHUNK = 4096; // max iterations per test in one single-workgroup launch
 constexpr uint32_t GRAPH_ITERS = 16;    // iterations per captured graph (multi-pass plan)
 
 uint64_t full_iters(uint32_t p, int mode) { return mode == M2_MODE_LL ? (p >= 2 ? p - 2 : 0) : p; }
@@ -688,18 +710,15 @@ int m2_test_time(m2_test* t, uint64_t iters, double* ms) {
   HIPCHK(hipSetDevice(t->ctx->device));
   int rc = test_sync(t);
   if (rc) return rc;
-  hipEvent_t a, b;
   if ((rc = t->group.prepare({t->obj}, t->graph_iters))) return rc;
-  HIPCHK(hipEventCreate(&a));
-  HIPCHK(hipEventCreate(&b));
-  HIPCHK(hipEventRecord(a, t->group.stream));
+  Events ev;
+  HIPCHK(ev.create(2));
+  HIPCHK(hipEventRecord(ev[0], t->group.stream));
   if ((rc = t->group.advance({t->obj}, iters, t->graph_iters))) return rc;
-  HIPCHK(hipEventRecord(b, t->group.stream));
-  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipEventRecord(ev[1], t->group.stream));
+  HIPCHK(hipEventSynchronize(ev[1]));
   float f = 0;
-  HIPCHK(hipEventElapsedTime(&f, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  HIPCHK(hipEventElapsedTime(&f, ev[0], ev[1]));
   *ms = f;
   return test_sync(t);
 }
@@ -714,11 +733,11 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
   // timestamps; per-kernel totals and launch counts give the mean per launch.
   constexpr uint64_t RUN = 16;
   constexpr int MAXE = 2 * (int)RUN + 8;
-  std::vector<hipEvent_t> starts(MAXE), stops(MAXE);
-  for (int k = 0; k < MAXE; ++k) {
-    HIPCHK(hipEventCreate(&starts[k]));
-    HIPCHK(hipEventCreate(&stops[k]));
-  }
+  Events ev_start, ev_stop;
+  HIPCHK(ev_start.create(MAXE));
+  HIPCHK(ev_stop.create(MAXE));
+  hipEvent_t* starts = ev_start.ev.data();
+  hipEvent_t* stops = ev_stop.ev.data();
   TestDesc* d = nullptr;
   std::vector<uint32_t> one(1, 1);
   if ((rc = t->group.upload_descs({t->obj}, &one, t->group.transient, &d))) return rc;
@@ -736,9 +755,9 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
       names[0] = "k_small";
     } else if (fused) {
       k = std::min<uint64_t>(RUN, iters - done);
-      HIPCHK(launch_run_fused_events(P, d, k, t->group.stream, starts.data(), stops.data(), MAXE, &nk, names));
+      HIPCHK(launch_run_fused_events(P, d, k, t->group.stream, starts, stops, MAXE, &nk, names));
     } else {
-      HIPCHK(launch_iteration_events(P, d, 1, t->group.stream, starts.data(), stops.data(), MAXE, &nk, names));
+      HIPCHK(launch_iteration_events(P, d, 1, t->group.stream, starts, stops, MAXE, &nk, names));
     }
     done += k;
     t->obj->iter += k;
@@ -751,10 +770,6 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
       acc[nm].first += f;
       acc[nm].second += 1;
     }
-  }
-  for (int k = 0; k < MAXE; ++k) {
-    (void)hipEventDestroy(starts[k]);
-    (void)hipEventDestroy(stops[k]);
   }
   if ((rc = test_sync(t))) return rc;
   const int m = std::min((int)order.size(), max_kernels);
@@ -946,8 +961,9 @@ int time_candidate(m2_ctx* ctx, uint64_t p, uint32_t ntests, const m2_opts& o, d
     t->target = ~0ull;
     ts.push_back(t);
   }
-  hipEvent_t a, b;
-  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return cleanup(fail(M2_E_HIP, "event"));
+  Events ev;
+  if (ev.create(2) != hipSuccess) return cleanup(fail(M2_E_HIP, "event"));
+  hipEvent_t a = ev[0], b = ev[1];
   auto run = [&](uint64_t k, float* ms) -> int {
     int r = g.prepare(ts, 0);
     if (r) return r;
@@ -966,8 +982,6 @@ int time_candidate(m2_ctx* ctx, uint64_t p, uint32_t ntests, const m2_opts& o, d
     if (ms >= 4.0f) break;
     k = std::min<uint64_t>(8192, (uint64_t)(k * std::max(2.0, 8.0 / std::max(ms, 0.01f))));
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   if (!rc) *us = ms * 1e3 / (double)k;
   return cleanup(rc);
 }
@@ -1302,8 +1316,11 @@ int server_admit(m2_server* s, bool* progressed) {
       m2_server::Lane lane;
       if ((rc = lane.g.create(cls))) return rc;
       if (o.graph_iters) lane.g.graph_iters = o.graph_iters;
-      HIPCHK(hipEventCreate(&lane.ev0));
-      HIPCHK(hipEventCreate(&lane.ev));
+      if (hipEventCreate(&lane.ev0) != hipSuccess || hipEventCreate(&lane.ev) != hipSuccess) {
+        if (lane.ev0) (void)hipEventDestroy(lane.ev0);
+        lane.g.destroy();
+        return fail(M2_E_HIP, "event");
+      }
       it = s->lanes.emplace(cls, lane).first;
     }
     TestObj* t = nullptr;
