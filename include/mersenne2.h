@@ -43,7 +43,9 @@ typedef struct m2_batch m2_batch;
 /* Tuning options; zero fields mean "automatic". The reference's -W, -CW, -B
  * and -LS flags (mersenne2_gpu.cpp:3660-3681) map onto these. */
 typedef struct m2_opts {
-  uint32_t wg;              /* -W: work-group size hint (the kernels use 256) */
+  uint32_t wg;              /* -W: workgroup size of the centre pass: 1024 (4 elements per
+                               thread, radix-4 steps) or 512 (8 per thread); other values leave
+                               the choice to the engine */
   uint32_t carry_wg;        /* -CW: carry work-group size hint (unused: carries run inside the passes) */
   uint32_t carry_pairs;     /* -B: digit pairs per carry block (a strip row of the outer pass),
                                rounded down to a power of two and clamped to the plan */

@@ -61,7 +61,7 @@ struct Events {
   hipEvent_t operator[](size_t i) const { return ev[i]; }
 };
 
-constexpr uint32_t SMALL_CHUNK = 4096; // max iterations per test in one single-workgroup launch
+constexpr uint32_t SMALL_CHUNK = 4096;  // max iterations per test in one single-workgroup launch
 constexpr uint32_t GRAPH_ITERS = 16;    // iterations per captured graph (multi-pass plan)
 
 uint64_t full_iters(uint32_t p, int mode) { return mode == M2_MODE_LL ? (p >= 2 ? p - 2 : 0) : p; }
@@ -73,7 +73,10 @@ uint64_t full_iters(uint32_t p, int mode) { return mode == M2_MODE_LL ? (p >= 2 
 //   n = 2^13: 39.1 -> 30.0 (centre 2^6, carry block 16)   2^14: 40.2 -> 32.3 (2^6, 16)
 //   n = 2^15: 44.0 -> 34.0 (centre 2^8, carry block 16)   2^16: 44.7 -> 36.4 (2^8, 16)
 //   n = 2^17: 46.1 -> 40.0 (centre 2^6)                   2^18: 50.0 -> 44.4 (2^8)
-//   n = 2^19: 53.8 -> 48.7 (centre 2^8)                   others: the default plan.
+//   n = 2^19: 53.8 -> 48.7 (centre 2^8)
+//   n = 2^20: 58.0 -> 56.5 (1024-thread centre: two blocks of 2^10 on 256
+//             workgroups, one per CU, leave a SIMD two 512-thread waves)
+//   others: the default plan.
 PlanOpts single_test_profile(int ln) {
   PlanOpts r;
   switch (ln) {
@@ -81,6 +84,7 @@ PlanOpts single_test_profile(int ln) {
     case 15: case 16: r.ls_cap = 512; r.carry_pairs = 16; break;
     case 17: r.ls_cap = 128; break;
     case 18: case 19: r.ls_cap = 512; break;
+    case 20: r.threads = MP_THREADS_WIDE; break;
     default: break;
   }
   return r;
@@ -99,6 +103,9 @@ PlanOpts plan_opts(const m2_opts* o) {
     r.ls_cap = o->local_stage_cap;
     r.carry_pairs = o->carry_pairs;
     r.max_radix = o->max_radix;
+    // -W 1024 or 512 picks the multi-pass kernels' workgroup size; other
+    // values leave it to the engine
+    if (o->wg == (uint32_t)MP_THREADS_WIDE || o->wg == (uint32_t)MP_THREADS_NARROW) r.threads = o->wg;
   }
   return r;
 }
@@ -594,7 +601,7 @@ int m2_test_create(m2_ctx* ctx, uint64_t p, int mode, const m2_opts* opts, m2_te
   if (ln < 3 || ln > 27) return fail(M2_E_RANGE, "exponent out of range (3 <= p < 2^32, n <= 2^27)");
   t->graph_iters = (opts && opts->graph_iters) ? opts->graph_iters : single_test_graph_iters(ln);
   PlanOpts po = plan_opts(opts);
-  if (!po.ls_cap && !po.carry_pairs && !po.max_radix) po = single_test_profile(ln);
+  if (!po.ls_cap && !po.carry_pairs && !po.max_radix && !po.threads) po = single_test_profile(ln);
   // an exponent whose digits are too narrow for the profile's multi-pass plan keeps the default
   {
     const uint32_t q = (uint32_t)(p >> ln);
@@ -929,8 +936,10 @@ uint32_t tune_bucket(uint32_t n) {
 
 std::string opts_str(const m2_opts& o) {
   char buf[128];
-  snprintf(buf, sizeof buf, "LS=%u B=%u radix=%u graph=%u", o.local_stage_cap, o.carry_pairs,
-           o.max_radix ? o.max_radix : 8u, o.graph_iters ? o.graph_iters : GRAPH_ITERS);
+  const bool wide = o.wg == (uint32_t)MP_THREADS_WIDE;
+  snprintf(buf, sizeof buf, "LS=%u B=%u radix=%u graph=%u%s", o.local_stage_cap, o.carry_pairs,
+           wide ? 4u : (o.max_radix ? o.max_radix : 8u), o.graph_iters ? o.graph_iters : GRAPH_ITERS,
+           wide ? " W=1024" : "");
   return buf;
 }
 
@@ -1017,6 +1026,12 @@ int autotune(m2_ctx* ctx, uint64_t p, uint32_t ntests, m2_opts* out, std::string
     add(2u << cl, 0, 8);
     add(2u << cl, 0, 4);
     add(2u << cl, 16, 8);
+  }
+  // one test whose centre grid leaves CUs with one workgroup: the 1024-thread centre
+  const uint32_t cus = (uint32_t)ctx->prop.multiProcessorCount;
+  if (ntests <= 1 && L >= CENTRE_LOG_C + 2 && (1u << (L - CENTRE_LOG_C - 1)) <= cus) {
+    add(2u << CENTRE_LOG_C, 0, 8);
+    cand.back().wg = MP_THREADS_WIDE;
   }
   const uint32_t nt = std::min<uint32_t>(std::max<uint32_t>(ntests, 1), 256);
   std::vector<std::pair<int, PlanOpts>> made;

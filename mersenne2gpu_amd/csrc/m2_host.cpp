@@ -106,36 +106,38 @@ Pass make_pass(int ell0, int A, std::vector<e61>& tw61, std::vector<e31>& tw31, 
 }
 
 // Register twiddles of centre thread t (either field), as k_centre keeps
-// them: slot k (r - 1) + e - 1 holds tau^e of the butterfly the thread takes
-// in iteration k of the last step (lds_step's thread -> butterfly map with one
-// column and two blocks), slot CENTRE_STEP_SLOTS + k the squaring twiddle of
-// the pair it takes in iteration k (square_self's map for workgroup 0,
-// square_cross's otherwise). Unused slots stay 0.
+// them with cs = centre_slots(nt): slot k (r - 1) + e - 1 holds tau^e of the
+// butterfly the thread takes in iteration k of the last step (lds_step's
+// thread -> butterfly map with one column and two blocks), slot
+// cs.step_slots + k the squaring twiddle of the pair it takes in iteration k
+// (square_self's map for workgroup 0, square_cross's otherwise). Unused slots
+// stay 0.
 template <class E>
-void centre_register_twiddles(const Pass& cp, uint32_t g, const uint32_t bl[2], const std::vector<E>& tw,
-                              const std::vector<E>& sq, std::vector<E>& out) {
+void centre_register_twiddles(const Pass& cp, const CentreSlots& cs, uint32_t g, const uint32_t bl[2],
+                              const std::vector<E>& tw, const std::vector<E>& sq, std::vector<E>& out) {
   const int c = cp.A;
   const Step& st = cp.steps[cp.nsteps - 1];
   const int r = st.radix, lr = log2i(r), lam = st.ell - cp.ell0;
   const int logd = c - lam - lr, logU = c - lr;
   const int total = 2 << logU;
   const int Cb = 1 << c, half = Cb >> 1;
+  const int rt = cs.role_threads;
   const size_t base = out.size();
-  out.resize(base + (size_t)CENTRE_TW_SLOTS * CENTRE_ROLE_THREADS, E{});
-  auto at = [&](int slot, int t) -> E& { return out[base + (size_t)slot * CENTRE_ROLE_THREADS + t]; };
-  for (int t = 0; t < CENTRE_ROLE_THREADS; ++t) {
-    for (int k = 0; t + k * CENTRE_ROLE_THREADS < total; ++k) {
-      const int bt = t + k * CENTRE_ROLE_THREADS;
+  out.resize(base + (size_t)(cs.step_slots + cs.sq_slots) * rt, E{});
+  auto at = [&](int slot, int t) -> E& { return out[base + (size_t)slot * rt + t]; };
+  for (int t = 0; t < rt; ++t) {
+    for (int k = 0; t + k * rt < total; ++k) {
+      const int bt = t + k * rt;
       const int b = bt >> logU, u = bt & ((1 << logU) - 1), blk = u >> logd;
       const uint32_t g0 = st.twoff + ((bl[b] << lam) + (uint32_t)blk) * (uint32_t)(r - 1);
       for (int e = 1; e < r; ++e) at(k * (r - 1) + e - 1, t) = tw[g0 + e - 1];
     }
-    for (int k = 0; k < CENTRE_SQ_SLOTS; ++k) {
-      const int o = t + k * CENTRE_ROLE_THREADS;
+    for (int k = 0; k < cs.sq_slots; ++k) {
+      const int o = t + k * rt;
       if (o >= Cb) break;  // both maps cover Cb pairs
       const int ti = g == 0 ? o : ((o & 1) == 0 ? (o >> 1) : half + ((Cb - 1 - o) >> 1));
       const int b = ti >= half ? 1 : 0;
-      at(CENTRE_STEP_SLOTS + k, t) = sq[((size_t)bl[b] << (c - 1)) + (size_t)(ti - b * half)];
+      at(cs.step_slots + k, t) = sq[((size_t)bl[b] << (c - 1)) + (size_t)(ti - b * half)];
     }
   }
 }
@@ -151,10 +153,16 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
     err = "max radix must be 4 or 8";
     return false;
   }
+  if (opts.threads != 0 && opts.threads != (uint32_t)MP_THREADS_NARROW && opts.threads != (uint32_t)MP_THREADS_WIDE) {
+    err = "workgroup size must be 512 or 1024";
+    return false;
+  }
+  const bool wide = opts.threads == (uint32_t)MP_THREADS_WIDE;
   const uint32_t ls_cap = opts.ls_cap;
   const int maxr = opts.max_radix ? (int)opts.max_radix : 8;
   const uint64_t n = uint64_t(1) << ln, h = n / 2;
   P = DevPlan{};
+  P.centre_nt = wide ? MP_THREADS_WIDE : MP_THREADS_NARROW;
   P.ln = ln;
   P.L = ln - 1;
   P.h = (uint32_t)h;
@@ -208,7 +216,8 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
       P.mid_logS = L - a_outer - a_mid;
       P.mid_logW = std::min(11 - a_mid, P.mid_logS);
     }
-    P.centre = make_pass(L - c, c, T.tw61, T.tw31, maxr);
+    // a 1024-thread centre holds 4 elements per thread: radix-4 steps
+    P.centre = make_pass(L - c, c, T.tw61, T.tw31, wide ? 4 : maxr);
     P.nseg = (uint32_t)(h >> P.logW);
   }
   // centre twiddles: w_h^bitrev(j) for position 2j (the reference's w[h/2+j])
@@ -251,13 +260,14 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
       nE += 2u * (1u << (cp.steps[si].ell - cp.ell0)) * (uint32_t)(cp.steps[si].radix - 1);
     const Step& last = cp.steps[cp.nsteps - 1];
     const int lr = log2i(last.radix);
-    const int per_thread = ((2 << (c - lr)) + CENTRE_ROLE_THREADS - 1) / CENTRE_ROLE_THREADS;
-    if (last.radix < 4 || nE > (uint32_t)CENTRE_EARLY_TW || per_thread * (last.radix - 1) > CENTRE_STEP_SLOTS ||
-        (2 << (c - 1)) > CENTRE_SQ_SLOTS * CENTRE_ROLE_THREADS) {
+    const CentreSlots cs = centre_slots(P.centre_nt);
+    const int per_thread = ((2 << (c - lr)) + cs.role_threads - 1) / cs.role_threads;
+    if (last.radix < 4 || nE > (uint32_t)CENTRE_EARLY_TW || per_thread * (last.radix - 1) > cs.step_slots ||
+        (2 << (c - 1)) > cs.sq_slots * cs.role_threads) {
       err = "plan error: centre twiddles do not fit the workgroup's slots";
       return false;
     }
-    P.crec = nE + (uint32_t)(CENTRE_TW_SLOTS * CENTRE_ROLE_THREADS);
+    P.crec = nE + (uint32_t)((cs.step_slots + cs.sq_slots) * cs.role_threads);
     T.ctw61.clear();
     T.ctw31.clear();
     T.ctw61.reserve((size_t)ncentre * P.crec);
@@ -275,8 +285,8 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
           T.ctw31.insert(T.ctw31.end(), T.tw31.begin() + g0, T.tw31.begin() + g0 + span);
         }
       }
-      centre_register_twiddles(cp, g, bl, T.tw61, T.sq61, T.ctw61);
-      centre_register_twiddles(cp, g, bl, T.tw31, T.sq31, T.ctw31);
+      centre_register_twiddles(cp, cs, g, bl, T.tw61, T.sq61, T.ctw61);
+      centre_register_twiddles(cp, cs, g, bl, T.tw31, T.sq31, T.ctw31);
     }
   }
   return true;
@@ -358,8 +368,9 @@ std::string describe_plan(const DevPlan& P) {
     for (int i = 0; i < ps.nsteps; ++i) o << (i ? "," : " ") << ps.steps[i].radix;
     o << "\n";
   };
-  o << "plan: h=" << P.h << " (n=" << 2ull * P.h << "), " << (P.small ? "single-workgroup LDS-resident" : "multi-pass")
-    << "\n";
+  o << "plan: h=" << P.h << " (n=" << 2ull * P.h << "), ";
+  if (P.small) o << "single-workgroup LDS-resident\n";
+  else o << "multi-pass\n";
   if (P.small) {
     pass("  k_small", P.outer);
   } else {
@@ -371,7 +382,8 @@ std::string describe_plan(const DevPlan& P) {
       o << "    columns S=" << (1u << P.mid_logS) << ", W=" << (1u << P.mid_logW) << "\n";
     }
     pass("  k_centre", P.centre);
-    o << "    " << ((P.h >> P.centre.A) / 2) << " workgroups of 2 blocks x " << (1u << P.centre.A) << "\n";
+    o << "    " << ((P.h >> P.centre.A) / 2) << " workgroups of 2 blocks x " << (1u << P.centre.A) << ", "
+      << P.centre_nt << " threads\n";
   }
   return o.str();
 }

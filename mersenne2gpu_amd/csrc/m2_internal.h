@@ -48,11 +48,13 @@ struct DevPlan {
   int mid_logS = 0, mid_logW = 0;
   Pass centre{};
   uint32_t nseg = 0;  // number of carry segments (length of G)
+  int centre_nt = MP_THREADS_NARROW;  // threads per centre workgroup (the strided passes use 512)
   // Centre twiddles regrouped per centre workgroup: record g (crec entries)
   // holds, for its blocks (B, B2), the twiddles of all but the last step in
-  // the order the workgroup keeps them in LDS, then CENTRE_TW_SLOTS slots of
-  // CENTRE_ROLE_THREADS entries: slot s, entry t is the twiddle thread t of
-  // either field keeps in register s (centre_register_twiddles).
+  // the order the workgroup keeps them in LDS, then the register slots of
+  // centre_slots(centre_nt), each of role_threads entries: slot s, entry t is
+  // the twiddle thread t of either field keeps in register s
+  // (centre_register_twiddles).
   const e61* ctw61 = nullptr;
   const e31* ctw31 = nullptr;
   uint32_t crec = 0;
@@ -137,13 +139,17 @@ struct TestTables {
 //   carry_pairs (-B)  digit pairs per strip row of the outer pass (the carry
 //               block), rounded down to a power of two and clamped to what
 //               the outer pass's row count allows;
-//   max_radix   8 (radix-8 steps) or 4 (radix-4 steps).
+//   max_radix   8 (radix-8 steps) or 4 (radix-4 steps);
+//   threads     (-W)  centre workgroup size: 512 (0: automatic) or 1024
+//               (4 elements per thread, radix-4 centre steps).
 struct PlanOpts {
   uint32_t ls_cap = 0, carry_pairs = 0, max_radix = 0;
+  uint32_t threads = 0;
   bool operator<(const PlanOpts& o) const {
     if (ls_cap != o.ls_cap) return ls_cap < o.ls_cap;
     if (carry_pairs != o.carry_pairs) return carry_pairs < o.carry_pairs;
-    return max_radix < o.max_radix;
+    if (max_radix != o.max_radix) return max_radix < o.max_radix;
+    return threads < o.threads;
   }
 };
 

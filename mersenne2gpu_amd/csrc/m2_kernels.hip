@@ -42,8 +42,14 @@ __device__ __forceinline__ int64_t to_i64(uint64_t x) { return (int64_t)x; }
 __device__ __forceinline__ int PADX(int i) { return i + (i >> 3); }
 constexpr int padded(int n) { return n + n / 8; }
 
-// Threads per workgroup of the multi-pass kernels: 256 per field.
-constexpr int MP_THREADS = 512;
+// Threads per workgroup of the strided (outer and middle) passes: 256 per
+// field. The centre pass takes its count NT as a template parameter:
+// MP_THREADS_NARROW or MP_THREADS_WIDE, half per field.
+constexpr int MP_THREADS = MP_THREADS_NARROW;
+template <int NT>
+constexpr int centre_tw_slots() {
+  return centre_slots(NT).step_slots + centre_slots(NT).sq_slots;
+}
 
 // Signed digit -> canonical field residues.
 __device__ __forceinline__ uint64_t digit61(int64_t d) { return d >= 0 ? (uint64_t)d : P61 - (uint64_t)(-d); }
@@ -123,11 +129,11 @@ __device__ __forceinline__ void store_carry(CP G, uint32_t i, i128 v) {
 
 // Carry segments: 4 elements (8 digits), first digit j0. In every kernel
 // that carries, each thread owns at most one segment (a workgroup holds at
-// most 2048 elements, 512 segments, with 512 threads) and keeps its 8 digits
+// most 2048 elements, 512 segments, with 512 or 1024 threads) and keeps its 8 digits
 // in registers from the normalisation to the weighting, so the digits do not
 // go through LDS between the carry steps.
-static_assert(PASS_MAX_ELEMS / SEG_ELEMS <= MP_THREADS, "one carry segment per thread");
-static_assert((1 << SMALL_LOG_H) / SEG_ELEMS <= MP_THREADS, "one carry segment per thread");
+static_assert(PASS_MAX_ELEMS / SEG_ELEMS <= MP_THREADS_NARROW, "one carry segment per thread");
+static_assert((1 << SMALL_LOG_H) / SEG_ELEMS <= MP_THREADS_NARROW, "one carry segment per thread");
 struct SegDigits {
   int64_t d[2 * SEG_ELEMS];
 };
@@ -217,11 +223,11 @@ __device__ __forceinline__ int stage_pass_twiddles(e61* T61, e31* T31, const Pas
                                                    const e61* __restrict__ tw61, const e31* __restrict__ tw31) {
   if (ps.ell0 == 0 && nb == 1 && j0[0] == 0) {
     // a pass that starts at level 0 uses its steps' whole tables, which lie
-    // back to back: one flat batched copy of 2^A - 1 entries
+    // back to back: one flat copy of 2^A - 1 entries
     const int n = (1 << ps.A) - 1;
     const e61* g61 = tw61 + ps.steps[0].twoff;
     const e31* g31 = tw31 + ps.steps[0].twoff;
-    gather2<4>(T61, T31, g61, g31, n, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+    gather2<1>(T61, T31, g61, g31, n, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
     return n;
   }
   int lo = 0;
@@ -476,12 +482,13 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, S61 
 // read from the global tables at twi(b, blk) + e - 1. A first step so needs
 // no barrier before it (each wave starts on its own loads as they arrive),
 // and a last step no LDS write-back and store loop after it.
-template <bool FWD, int RADIX, bool LOAD, class GI, class TI>
+template <int NT, bool FWD, int RADIX, bool LOAD, class GI, class TI>
 __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW, int nb, int lam,
                                           M2_GLOBAL e61* z61, M2_GLOBAL e31* z31, GI gidx,
                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31, TI twi) {
   constexpr int LR = RADIX == 8 ? 3 : (RADIX == 4 ? 2 : 1);
-  constexpr int KMAX = PASS_MAX_ELEMS / RADIX / (MP_THREADS / 2);  // butterflies per thread
+  // butterflies per thread (radix 8 on 1024 threads: one, on half the threads)
+  constexpr int KMAX = PASS_MAX_ELEMS / RADIX / (NT / 2) > 0 ? PASS_MAX_ELEMS / RADIX / (NT / 2) : 1;
   const Lanes<true> ln;
   const int logd = logR - lam - LR;
   const int logU = logR - LR + logW;
@@ -580,21 +587,21 @@ __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW
 
 // edge_step for step si of pass ps (radix dispatch). No barrier: the caller
 // adds it where the LDS results are read by other threads.
-template <bool FWD, bool LOAD, class GI, class TI>
+template <int NT, bool FWD, bool LOAD, class GI, class TI>
 __device__ __forceinline__ void edge_pass_step(e61* L61, e31* L31, int logW, int nb, const Pass& ps, int si,
                                                M2_GLOBAL e61* z61, M2_GLOBAL e31* z31, GI gidx,
                                                const e61* tw61, const e31* tw31, TI twi) {
   const Step st = ps.steps[si];
   const int lam = st.ell - ps.ell0;
-  if (st.radix == 8) edge_step<FWD, 8, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
-  else if (st.radix == 4) edge_step<FWD, 4, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
-  else edge_step<FWD, 2, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
+  if (st.radix == 8) edge_step<NT, FWD, 8, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
+  else if (st.radix == 4) edge_step<NT, FWD, 4, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
+  else edge_step<NT, FWD, 2, LOAD>(L61, L31, ps.A, logW, nb, lam, z61, z31, gidx, tw61, tw31, twi);
 }
 
 // tw[first + k] for a runtime k < N, by selects, so that the register array
 // is never indexed dynamically (which would move it to scratch memory).
-template <int N>
-__device__ __forceinline__ e61 pick_slot(const e61 (&tw)[CENTRE_TW_SLOTS], int first, int k) {
+template <int N, int NS>
+__device__ __forceinline__ e61 pick_slot(const e61 (&tw)[NS], int first, int k) {
   e61 r = tw[first];
 #pragma unroll
   for (int j = 1; j < N; ++j) {
@@ -612,12 +619,14 @@ __device__ __forceinline__ e61 pick_slot(const e61 (&tw)[CENTRE_TW_SLOTS], int f
 // thread's twiddles in registers: iteration k's butterfly uses slots
 // k (r - 1) .. k (r - 1) + r - 2 (M61 lanes: tw[s]; M31 lanes: the e31
 // packed in tw[s].a). One column, nb blocks, as lds_step with logW = 0.
-// Radix 8 (one butterfly per thread) or 4 (at most two).
-template <bool FWD, int RADIX>
+// Radix 8 (one butterfly per thread) or 4 (at most two on 512 threads, one
+// on 1024).
+template <bool FWD, int RADIX, int NT>
 __device__ __forceinline__ void reg_step(e61* L61, e31* L31, int logR, int nb, int lam,
-                                         const e61 (&tw)[CENTRE_TW_SLOTS]) {
+                                         const e61 (&tw)[centre_tw_slots<NT>()]) {
   constexpr int LR = RADIX == 8 ? 3 : (RADIX == 4 ? 2 : 1);
-  constexpr int KMAX = CENTRE_STEP_SLOTS / (RADIX - 1);
+  constexpr int KMAX = centre_slots(NT).step_slots / (RADIX - 1);
+  static_assert(KMAX >= 1, "a thread's last-step twiddles fit its slots");
   const Lanes<true> ln;
   const int logd = logR - lam - LR;
   const int logU = logR - LR;
@@ -688,15 +697,20 @@ __device__ __forceinline__ void reg_step(e61* L61, e31* L31, int logR, int nb, i
   __syncthreads();
 }
 
-template <bool FWD>
+template <bool FWD, int NT>
 __device__ __forceinline__ void centre_last_step(e61* L61, e31* L31, const Pass& ps, int nb,
-                                                 const e61 (&tw)[CENTRE_TW_SLOTS]) {
+                                                 const e61 (&tw)[centre_tw_slots<NT>()]) {
   const Step st = ps.steps[ps.nsteps - 1];
   const int lam = st.ell - ps.ell0;
-  // the centre has at least 2 levels, so its last step is radix 8 or 4 (the
-  // host refuses other plans)
-  if (st.radix == 8) reg_step<FWD, 8>(L61, L31, ps.A, nb, lam, tw);
-  else reg_step<FWD, 4>(L61, L31, ps.A, nb, lam, tw);
+  // the centre has at least 2 levels, so its last step is radix 8 or 4, and
+  // radix 4 on 1024 threads (the host refuses other plans)
+  if constexpr (NT == MP_THREADS_NARROW) {
+    if (st.radix == 8) {
+      reg_step<FWD, 8, NT>(L61, L31, ps.A, nb, lam, tw);
+      return;
+    }
+  }
+  reg_step<FWD, 4, NT>(L61, L31, ps.A, nb, lam, tw);
 }
 
 // Copies count elements of both fields, global index src(i) -> LDS index
@@ -900,7 +914,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31, 0, edges ? ns - 1 : ns);
   if (edges) {
     const Step sl = ps.steps[ns - 1];
-    edge_pass_step<true, false>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31,
+    edge_pass_step<MP_THREADS, true, false>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31,
                                 [&](int, int blk) { return sl.twoff + (uint32_t)blk * (uint32_t)(sl.radix - 1); });
     return;
   }
@@ -1027,7 +1041,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   const bool edges = ns >= 2;
   const Step sl = ps.steps[ns - 1];
   auto twl = [&](int, int blk) { return sl.twoff + (uint32_t)blk * (uint32_t)(sl.radix - 1); };
-  if (edges) edge_pass_step<false, true>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
+  if (edges) edge_pass_step<MP_THREADS, false, true>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
   else gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
@@ -1066,7 +1080,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   __syncthreads();
   lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31, 0, edges ? ns - 1 : ns);
   if (edges) {
-    edge_pass_step<true, false>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
+    edge_pass_step<MP_THREADS, true, false>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
     return;
   }
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
@@ -1113,7 +1127,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
 // last step are staged in LDS; each thread loads the twiddles of its own
 // last-step butterflies and squaring pairs straight into registers, so the
 // workgroup needs 66 KB of LDS and two fit on a CU.
-__global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
+template <int NT>
+__global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
                                                 const e31* __restrict__ ctw31, uint32_t crec, Pass ps,
                                                 int direct_in) {
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
@@ -1147,41 +1162,43 @@ __global__ void __launch_bounds__(MP_THREADS) k_centre(const TestDesc* __restric
   auto tw0 = [&](int b, int blk) {
     return (uint32_t)(b * twiddle_span(s0, s0.ell - ps.ell0) + blk * (s0.radix - 1));
   };
-  if (edge_in) edge_pass_step<true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
-  else gather2<4>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
+  if (edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
+  else gather2<PASS_MAX_ELEMS / NT>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
   int nE = 0;
   for (int si = 0; si + 1 < ns; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
   gather2<1>(T61, T31, r61, r31, nE, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
-  // register twiddles: slot s of thread t at record entry nE + s * 256 + t
-  // (coalesced); M31 lanes keep their e31 packed in .a
+  // register twiddles: slot s of thread t at record entry nE + s * RT + t,
+  // RT threads per field (coalesced); M31 lanes keep their e31 packed in .a
+  constexpr CentreSlots CS = centre_slots(NT);
+  constexpr int NSLOT = centre_tw_slots<NT>();
   const Lanes<true> ln;
-  e61 tw[CENTRE_TW_SLOTS];
+  e61 tw[NSLOT];
   if (ln.f61) {
     const M2_GLOBAL e61* src = gptr(r61 + nE) + ln.tid;
 #pragma unroll
-    for (int k = 0; k < CENTRE_TW_SLOTS; ++k) tw[k] = gld(src + k * CENTRE_ROLE_THREADS);
+    for (int k = 0; k < NSLOT; ++k) tw[k] = gld(src + k * CS.role_threads);
   } else {
     const M2_GLOBAL e31* src = gptr(r31 + nE) + ln.tid;
 #pragma unroll
-    for (int k = 0; k < CENTRE_TW_SLOTS; ++k) {
-      const e31 w = gld(src + k * CENTRE_ROLE_THREADS);
+    for (int k = 0; k < NSLOT; ++k) {
+      const e31 w = gld(src + k * CS.role_threads);
       tw[k] = e61{(uint64_t)w.a | ((uint64_t)w.b << 32), 0};
     }
   }
   __syncthreads();
   lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, edge_in ? 1 : 0, ns - 1);
-  centre_last_step<true>(L61, L31, ps, nb, tw);
-  auto s61 = [&](int k, int) { return pick_slot<CENTRE_SQ_SLOTS>(tw, CENTRE_STEP_SLOTS, k); };
+  centre_last_step<true, NT>(L61, L31, ps, nb, tw);
+  auto s61 = [&](int k, int) { return pick_slot<CS.sq_slots>(tw, CS.step_slots, k); };
   auto s31 = [&](int k, int) {
-    const e61 w = pick_slot<CENTRE_SQ_SLOTS>(tw, CENTRE_STEP_SLOTS, k);
+    const e61 w = pick_slot<CS.sq_slots>(tw, CS.step_slots, k);
     return e31{(uint32_t)w.a, (uint32_t)(w.a >> 32)};
   };
   if (g == 0) square_self<true>(L61, L31, logC, 2, s61, s31);
   else square_cross<true>(L61, L31, logC, s61, s31);
-  centre_last_step<false>(L61, L31, ps, nb, tw);
+  centre_last_step<false, NT>(L61, L31, ps, nb, tw);
   lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31, 1, edges ? ns - 1 : ns);
   if (edges) {
-    edge_pass_step<false, false>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
+    edge_pass_step<NT, false, false>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
     return;
   }
   for (int i = threadIdx.x; i < (nb << logC); i += blockDim.x) {
@@ -1206,60 +1223,74 @@ uint32_t g_fused_spin_limit = 1u << 20;
 
 namespace {
 // Launches the kernels of one iteration of every test in `tests` through
-// launch(name, kernel, grid, args...); grid.y indexes the tests.
-template <class F>
-hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& launch) {
+// launch(name, kernel, grid, block, args...); grid.y indexes the tests. NT is
+// the centre's workgroup size, P.centre_nt.
+template <int NT, class F>
+hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& launch) {
   hipError_t e;
   const uint32_t ngroups_outer = (1u << P.logS) >> P.logW;
-  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ngroups_outer, nt), tests, P.tw61, P.tw31, P.outer, P.logS,
+  const dim3 blk(MP_THREADS), cblk(NT);
+  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
                   P.logW)) != hipSuccess)
     return e;
   const uint32_t nmid = P.has_mid ? (1u << P.outer.A) * ((1u << P.mid_logS) >> P.mid_logW) : 0;
-  if (P.has_mid && (e = launch("k_mid_fwd", k_mid<true>, dim3(nmid, nt), tests, P.tw61, P.tw31, P.mid, P.mid_logS,
+  if (P.has_mid && (e = launch("k_mid_fwd", k_mid<true>, dim3(nmid, nt), blk, tests, P.tw61, P.tw31, P.mid, P.mid_logS,
                                P.mid_logW)) != hipSuccess)
     return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre * nt <= P.cus ? 1 : 0;
-  if ((e = launch("k_centre", k_centre, dim3(ncentre, nt), tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) !=
-      hipSuccess)
+  if ((e = launch("k_centre", k_centre<NT>, dim3(ncentre, nt), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
+                  direct_in)) != hipSuccess)
     return e;
-  if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid, nt), tests, P.tw61, P.tw31, P.mid, P.mid_logS,
+  if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid, nt), blk, tests, P.tw61, P.tw31, P.mid, P.mid_logS,
                                P.mid_logW)) != hipSuccess)
     return e;
-  return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer, nt), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
+  return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
+                P.logW);
+}
+template <class F>
+hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& launch) {
+  if (P.centre_nt == MP_THREADS_WIDE) return iteration_nt<MP_THREADS_WIDE>(P, tests, nt, launch);
+  return iteration_nt<MP_THREADS_NARROW>(P, tests, nt, launch);
 }
 
 
 // k >= 1 iterations of a single test (no middle pass) with the outer passes of
 // consecutive iterations fused: the run starts and ends with the digits.
-template <class F>
-hipError_t run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& launch) {
+template <int NT, class F>
+hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& launch) {
   hipError_t e;
   const uint32_t ns = outer_strips(P);
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre <= P.cus ? 1 : 0;
-  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
+  const dim3 blk(MP_THREADS), cblk(NT);
+  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
       hipSuccess)
     return e;
   for (uint64_t i = 0; i + 1 < k; ++i) {
-    if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) !=
-        hipSuccess)
+    if ((e = launch("k_centre", k_centre<NT>, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
+                    direct_in)) != hipSuccess)
       return e;
-    if ((e = launch("k_outer_fused", k_outer_fused, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW,
+    if ((e = launch("k_outer_fused", k_outer_fused, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW,
                     g_fused_spin_limit)) != hipSuccess)
       return e;
   }
-  if ((e = launch("k_centre", k_centre, dim3(ncentre, 1), tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) !=
-      hipSuccess)
+  if ((e = launch("k_centre", k_centre<NT>, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
+                  direct_in)) != hipSuccess)
     return e;
-  return launch("k_outer_inv", k_outer_inv, dim3(ns, 1), tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
+  return launch("k_outer_inv", k_outer_inv, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
+}
+template <class F>
+hipError_t run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& launch) {
+  if (P.centre_nt == MP_THREADS_WIDE) return run_fused_nt<MP_THREADS_WIDE>(P, tests, k, launch);
+  return run_fused_nt<MP_THREADS_NARROW>(P, tests, k, launch);
 }
 }  // namespace
 
 hipError_t launch_run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, hipStream_t st) {
   if (P.small || P.has_mid || k == 0) return hipErrorInvalidValue;
-  return run_fused(P, tests, k, [&](const char*, auto kernel, dim3 grid, auto... args) -> hipError_t {
-    hipLaunchKernelGGL(kernel, grid, dim3(MP_THREADS), 0, st, args...);
+  return run_fused(P, tests, k, [&](const char*, auto kernel, dim3 grid, dim3 block, auto... args) -> hipError_t {
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
     return hipGetLastError();
   });
 }
@@ -1269,10 +1300,10 @@ hipError_t launch_run_fused_events(const DevPlan& P, const TestDesc* tests, uint
                                    const char** names) {
   if (P.small || P.has_mid || k == 0) return hipErrorInvalidValue;
   int n = 0;
-  hipError_t e = run_fused(P, tests, k, [&](const char* name, auto kernel, dim3 grid, auto... args) -> hipError_t {
+  hipError_t e = run_fused(P, tests, k, [&](const char* name, auto kernel, dim3 grid, dim3 block, auto... args) -> hipError_t {
     if (n >= max_events) return hipErrorInvalidValue;
     names[n] = name;
-    hipExtLaunchKernelGGL(kernel, grid, dim3(MP_THREADS), 0, st, starts[n], stops[n], 0, args...);
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, starts[n], stops[n], 0, args...);
     ++n;
     return hipGetLastError();
   });
@@ -1281,8 +1312,8 @@ hipError_t launch_run_fused_events(const DevPlan& P, const TestDesc* tests, uint
 }
 
 hipError_t launch_iteration(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
-  return iteration(P, tests, ntests, [&](const char*, auto kernel, dim3 grid, auto... args) -> hipError_t {
-    hipLaunchKernelGGL(kernel, grid, dim3(MP_THREADS), 0, st, args...);
+  return iteration(P, tests, ntests, [&](const char*, auto kernel, dim3 grid, dim3 block, auto... args) -> hipError_t {
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, args...);
     return hipGetLastError();
   });
 }
@@ -1291,10 +1322,10 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
                                    hipEvent_t* starts, hipEvent_t* stops, int max_events, int* count,
                                    const char** names) {
   int k = 0;
-  hipError_t e = iteration(P, tests, ntests, [&](const char* name, auto kernel, dim3 grid, auto... args) -> hipError_t {
+  hipError_t e = iteration(P, tests, ntests, [&](const char* name, auto kernel, dim3 grid, dim3 block, auto... args) -> hipError_t {
     if (k >= max_events) return hipErrorInvalidValue;
     names[k] = name;
-    hipExtLaunchKernelGGL(kernel, grid, dim3(MP_THREADS), 0, st, starts[k], stops[k], 0, args...);
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, starts[k], stops[k], 0, args...);
     ++k;
     return hipGetLastError();
   });

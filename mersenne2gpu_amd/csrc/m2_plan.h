@@ -22,14 +22,25 @@ constexpr int SMALL_LOG_H = 11;     // h <= 2048 elements (48 KB of data in LDS)
 constexpr int CENTRE_LOG_C = 10;    // centre block: 1024 elements, two per workgroup
 constexpr int PASS_MAX_ELEMS = 2048;  // R * W elements per strided-pass workgroup
 constexpr int PASS_MAX_LOG_R = 9;     // strided passes: at most 2^9 rows (R - 1 twiddles)
-// Centre workgroups: 256 threads per field. Each thread keeps in registers
-// the twiddles of its butterflies in the last forward step (the first inverse
-// step uses them conjugated) and of its squaring pairs; only the other steps'
-// twiddles are staged in LDS.
-constexpr int CENTRE_ROLE_THREADS = 256;
-constexpr int CENTRE_STEP_SLOTS = 7;   // last step radix 8: 1 x 7, radix 4: 2 x 3
-constexpr int CENTRE_SQ_SLOTS = 4;     // 2^c <= 4 x 256 squaring pairs
-constexpr int CENTRE_TW_SLOTS = CENTRE_STEP_SLOTS + CENTRE_SQ_SLOTS;
+// Threads per workgroup of the multi-pass kernels, half per field: 512 (8
+// elements per thread of a 2048-element workgroup, radix-8 steps), or, for
+// the centre pass, 1024 (4 elements per thread, radix-4 steps: twice the
+// waves per SIMD, for a grid of at most one workgroup per CU, which would
+// leave a SIMD two waves).
+constexpr int MP_THREADS_NARROW = 512;
+constexpr int MP_THREADS_WIDE = 1024;
+// Centre workgroups: each thread keeps in registers the twiddles of its
+// butterflies in the last forward step (the first inverse step uses them
+// conjugated) and of its squaring pairs; only the other steps' twiddles are
+// staged in LDS. Slots per thread, for nt threads per workgroup:
+//   512: last step radix 8: 1 x 7 or radix 4: 2 x 3, then 2^c <= 4 x 256 pairs
+//   1024: last step radix 4: 1 x 3, then 2^c <= 2 x 512 pairs
+struct CentreSlots {
+  int role_threads, step_slots, sq_slots;
+};
+constexpr CentreSlots centre_slots(int nt) {
+  return nt == MP_THREADS_WIDE ? CentreSlots{512, 3, 2} : CentreSlots{256, 7, 4};
+}
 constexpr int CENTRE_EARLY_TW = 512;   // the other steps' twiddles, both blocks
 constexpr int SEG_ELEMS = 4;        // carry segment: 4 elements = 8 digits
 

@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <random>
 #include <set>
 #include <vector>
@@ -122,6 +123,60 @@ int main() {
       if (P.has_mid)
         CHECK(P.mid.A <= PASS_MAX_LOG_R && (1 << (P.mid.A + P.mid_logW)) <= PASS_MAX_ELEMS, "ln=%d mid tile", c.ln);
     }
+  }
+  // 1024-thread centre: radix-4 centre steps, radix-8 strided passes, and
+  // per-workgroup register records holding the same squaring twiddles as the
+  // 512-thread layout (each workgroup squares the same pairs; only the thread
+  // that takes a pair differs)
+  {
+    DevPlan Pn, Pw;
+    ClassTables Cn, Cw;
+    std::string err;
+    PlanOpts pn, pw;
+    pw.threads = MP_THREADS_WIDE;
+    const bool okn = build_class_plan(20, pn, Pn, Cn, err);
+    const bool okw = build_class_plan(20, pw, Pw, Cw, err);
+    CHECK(okn && okw, "ln=20 plans: %s", err.c_str());
+    if (okn && okw) {
+      CHECK(Pn.centre_nt == MP_THREADS_NARROW && Pw.centre_nt == MP_THREADS_WIDE, "centre sizes");
+      for (int k = 0; k < Pw.centre.nsteps; ++k) CHECK(Pw.centre.steps[k].radix == 4, "wide centre radix");
+      for (int k = 0; k < Pw.outer.nsteps; ++k)
+        CHECK(Pw.outer.steps[k].radix == Pn.outer.steps[k].radix, "outer steps unchanged");
+      const CentreSlots cn = centre_slots(MP_THREADS_NARROW), cw = centre_slots(MP_THREADS_WIDE);
+      auto early = [](const DevPlan& P) {
+        uint32_t nE = 0;
+        for (int si = 0; si + 1 < P.centre.nsteps; ++si)
+          nE += 2u * (1u << (P.centre.steps[si].ell - P.centre.ell0)) * (uint32_t)(P.centre.steps[si].radix - 1);
+        return nE;
+      };
+      const uint32_t nEn = early(Pn), nEw = early(Pw);
+      CHECK(Pw.crec == nEw + (uint32_t)((cw.step_slots + cw.sq_slots) * cw.role_threads), "wide record size");
+      const uint32_t ncentre = (Pw.h >> Pw.centre.A) / 2;
+      CHECK(Cw.ctw61.size() == (size_t)ncentre * Pw.crec && Cw.ctw31.size() == Cw.ctw61.size(), "wide records");
+      for (uint32_t g : {0u, 1u, 2u, 77u, ncentre - 1}) {
+        std::vector<std::pair<uint64_t, uint64_t>> a, b;
+        std::vector<uint64_t> a31, b31;
+        const size_t sn = (size_t)g * Pn.crec + nEn + (size_t)cn.step_slots * cn.role_threads;
+        const size_t sw = (size_t)g * Pw.crec + nEw + (size_t)cw.step_slots * cw.role_threads;
+        for (int i = 0; i < cn.sq_slots * cn.role_threads; ++i) {
+          a.push_back({Cn.ctw61[sn + i].a, Cn.ctw61[sn + i].b});
+          a31.push_back((uint64_t)Cn.ctw31[sn + i].a << 32 | Cn.ctw31[sn + i].b);
+        }
+        for (int i = 0; i < cw.sq_slots * cw.role_threads; ++i) {
+          b.push_back({Cw.ctw61[sw + i].a, Cw.ctw61[sw + i].b});
+          b31.push_back((uint64_t)Cw.ctw31[sw + i].a << 32 | Cw.ctw31[sw + i].b);
+        }
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        std::sort(a31.begin(), a31.end());
+        std::sort(b31.begin(), b31.end());
+        CHECK(a == b && a31 == b31, "g=%u squaring twiddles differ between centre sizes", g);
+      }
+    }
+    PlanOpts bad;
+    bad.threads = 256;
+    CHECK(!build_class_plan(20, bad, Pw, Cw, err) && err.find("workgroup size") != std::string::npos,
+          "workgroup size 256 refused: %s", err.c_str());
   }
   printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
   return fails ? 1 : 0;

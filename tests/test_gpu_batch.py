@@ -161,10 +161,12 @@ def test_server_one_request_larger_than_budget_runs_alone(gpu_ctx):
         assert out[b] == c["checkpoints"]["200"]["res64"]
 
 
-@pytest.mark.parametrize("variant", [dict(carry_pairs=16), dict(max_radix=4), dict(max_radix=4, carry_pairs=4)])
+@pytest.mark.parametrize("variant", [dict(carry_pairs=16), dict(max_radix=4), dict(max_radix=4, carry_pairs=4),
+                                     dict(wg=1024)])
 def test_plan_variants_match(gpu_ctx, variant):
-    """Carry-block width and radix choices (what the autotuner varies) leave
-    every residue unchanged, on the single-workgroup and multi-pass plans."""
+    """Carry-block width, radix and centre workgroup size (what the autotuner
+    varies) leave every residue unchanged, on the single-workgroup and
+    multi-pass plans."""
     for c in _sweep_cases():
         if not (5 <= c["ln"] <= 17 and c["mode"] == "prp"):
             continue

@@ -17,12 +17,12 @@ from conftest import load_fixture
 pytestmark = pytest.mark.gpu
 
 
-def _check(ctx, case, ls=0, max_k=None, graph_iters=0):
+def _check(ctx, case, ls=0, max_k=None, graph_iters=0, **opts):
     ks = sorted(int(k) for k in case["checkpoints"])
     if max_k is not None:
         ks = [k for k in ks if k <= max_k]
-    print("case p=%d %s ls=%d ks=%s" % (case["p"], case["mode"], ls, ks), flush=True)
-    t = m2.Test(ctx, case["p"], case["mode"], graph_iters=graph_iters) if not ls else _test_ls(ctx, case, ls)
+    print("case p=%d %s ls=%d ks=%s %s" % (case["p"], case["mode"], ls, ks, opts), flush=True)
+    t = m2.Test(ctx, case["p"], case["mode"], graph_iters=graph_iters, **opts) if not ls else _test_ls(ctx, case, ls)
     with t:
         done = 0
         for k in ks:
@@ -82,6 +82,16 @@ def test_c1_on_multipass_plan(gpu_ctx):
 def test_c2_checkpoints_n2p20(gpu_ctx):
     for c in load_fixture("c2.json")["cases"]:
         _check(gpu_ctx, c)
+
+
+def test_c2_centre_workgroup_sizes(gpu_ctx):
+    """The 512-thread (radix-8) and 1024-thread (radix-4) centre give the
+    same residues; the plan text names the size in use."""
+    c = load_fixture("c2.json")["cases"][0]
+    for wg in (512, 1024):
+        _check(gpu_ctx, c, max_k=1000, wg=wg)
+        with m2.Test(gpu_ctx, c["p"], c["mode"], wg=wg) as t:
+            assert "2 blocks x 1024, %d threads" % wg in t.plan(), t.plan()
 
 
 def test_c2_graph_and_direct_launches_agree(gpu_ctx):
