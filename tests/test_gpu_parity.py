@@ -86,12 +86,15 @@ def test_c2_checkpoints_n2p20(gpu_ctx):
 
 def test_c2_centre_workgroup_sizes(gpu_ctx):
     """The 512-thread (radix-8) and 1024-thread (radix-4) centre give the
-    same residues; the plan text names the size in use."""
+    same residues; the plan text names the size in use, and a single test
+    at this length takes the 1024-thread centre by default."""
     c = load_fixture("c2.json")["cases"][0]
     for wg in (512, 1024):
         _check(gpu_ctx, c, max_k=1000, wg=wg)
         with m2.Test(gpu_ctx, c["p"], c["mode"], wg=wg) as t:
             assert "2 blocks x 1024, %d threads" % wg in t.plan(), t.plan()
+    with m2.Test(gpu_ctx, c["p"], c["mode"]) as t:
+        assert "2 blocks x 1024, 1024 threads" in t.plan(), t.plan()
 
 
 def test_c2_graph_and_direct_launches_agree(gpu_ctx):
