@@ -70,10 +70,10 @@ uint64_t full_iters(uint32_t p, int mode) { return mode == M2_MODE_LL ? (p >= 2 
 // zero (the counterpart of the reference's per-device profile table,
 // mersenne2_gpu.cpp:3515-3616): m2_autotune's choices for one test, measured
 // with the fused outer pass (us/iteration, default plan -> chosen):
-//   n = 2^13: 39.1 -> 30.0 (centre 2^6, carry block 16)   2^14: 40.2 -> 32.3 (2^6, 16)
-//   n = 2^15: 44.0 -> 34.0 (centre 2^8, carry block 16)   2^16: 44.7 -> 36.4 (2^8, 16)
-//   n = 2^17: 46.1 -> 40.0 (centre 2^6)                   2^18: 50.0 -> 44.4 (2^8)
-//   n = 2^19: 53.8 -> 48.7 (centre 2^8)
+//   n = 2^13: 39.1 -> 30.0 (centre blocks 2^7, carry block 16)   2^14: 40.2 -> 32.3 (2^7, 16)
+//   n = 2^15: 44.0 -> 34.0 (centre blocks 2^9, carry block 16)   2^16: 44.7 -> 36.4 (2^9, 16)
+//   n = 2^17: 46.1 -> 40.0 (centre blocks 2^7)                   2^18: 50.0 -> 44.4 (2^9)
+//   n = 2^19: 53.8 -> 48.7 (centre blocks 2^9)
 //   n = 2^20: 58.0 -> 56.5 (1024-thread centre: two blocks of 2^10 on 256
 //             workgroups, one per CU, leave a SIMD two 512-thread waves)
 //   others: the default plan.

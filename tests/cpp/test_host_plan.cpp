@@ -27,7 +27,8 @@ int main() {
   std::mt19937_64 rng(42);
   const uint32_t ps[] = {3, 5, 7, 127, 349, 353, 607, 1279, 19937, 20983, 21001, 132049, 2424827, 27899687};
   const uint32_t caps[] = {0, 4, 16, 64, 256};
-  const uint32_t variants[][2] = {{0, 8}, {16, 8}, {0, 4}, {4, 4}};  // (carry_pairs, max_radix)
+  // (carry_pairs, max_radix, centre workgroup size)
+  const uint32_t variants[][3] = {{0, 8, 0}, {16, 8, 0}, {0, 4, 0}, {4, 4, 0}, {0, 8, 1024}};
   for (uint32_t p : ps) {
     for (uint32_t cap : caps) for (const auto& var : variants) {
       DevPlan P;
@@ -39,6 +40,7 @@ int main() {
       po.ls_cap = cap;
       po.carry_pairs = var[0];
       po.max_radix = var[1];
+      po.threads = var[2];
       const bool okc = build_class_plan(transform_log2(p), po, P, CT, err);
       CHECK(okc, "p=%u cap=%u class plan: %s", p, cap, err.c_str());
       if (!okc) continue;
@@ -76,7 +78,11 @@ int main() {
         CHECK(lv == ps->A, "p=%u levels %d != %d", p, lv, ps->A);
       }
       if (!P.small) CHECK((1 << P.logW) >= SEG_ELEMS && (P.outer.A + P.logW) <= 11, "p=%u outer tile", p);
-      if (cap == 0 && var[0] == 0 && var[1] == 8 && n <= (1u << 16)) {
+      if (!P.small && var[2]) {
+        for (int k = 0; k < P.centre.nsteps; ++k) CHECK(P.centre.steps[k].radix == 4, "p=%u wide centre", p);
+        CHECK(P.centre_nt == MP_THREADS_WIDE, "p=%u centre size", p);
+      }
+      if (cap == 0 && var[0] == 0 && var[1] == 8 && var[2] == 0 && n <= (1u << 16)) {
         std::vector<int64_t> d(n);
         for (uint64_t j = 0; j < n; ++j) d[j] = (int64_t)(rng() % (1ull << T.dig[j].width));
         std::vector<uint8_t> r = residue_from_state(p, T.bitpos, d.data(), n, nullptr, nullptr, 0);

@@ -178,6 +178,21 @@ def test_plan_variants_match(gpu_ctx, variant):
                 assert t.res64() == c["checkpoints"]["50"]["res64"], (c["p"], ls, variant)
 
 
+@pytest.mark.parametrize("ls", [4, 8, 16, 256])
+def test_wide_centre_on_small_blocks(gpu_ctx, ls):
+    """The 1024-thread centre on centre blocks of 2^2..2^8 points, where most
+    of its threads have no butterfly or squaring pair, gives the exact
+    residues."""
+    cl = max(2, ls.bit_length() - 1)  # centre blocks of 2^cl <= ls points; multi-pass from n = 2^(cl + 3)
+    for c in _sweep_cases():
+        if not (cl + 3 <= c["ln"] <= 14 and c["mode"] == "prp") or (c["p"] >> c["ln"]) < 12:
+            continue
+        with m2.Test(gpu_ctx, c["p"], "prp", local_stage_cap=ls, wg=1024) as t:
+            assert "%d threads" % 1024 in t.plan(), t.plan()
+            t.run(50)
+            assert t.res64() == c["checkpoints"]["50"]["res64"], (c["p"], ls)
+
+
 def test_autotune_and_tuned_batch(gpu_ctx):
     """The tuner reports every candidate and picks one; a batch built with
     autotune=1 gives the exact results."""
