@@ -133,6 +133,8 @@ __device__ __forceinline__ void store_carry(CP G, uint32_t i, i128 v) {
 // in registers from the normalisation to the weighting, so the digits do not
 // go through LDS between the carry steps.
 static_assert(PASS_MAX_ELEMS / SEG_ELEMS <= MP_THREADS_NARROW, "one carry segment per thread");
+static_assert((1 << PASS_MAX_LOG_R) - 1 <= MP_THREADS_NARROW && CENTRE_EARLY_TW <= MP_THREADS_NARROW,
+              "one staged twiddle per thread");
 static_assert((1 << SMALL_LOG_H) / SEG_ELEMS <= MP_THREADS_NARROW, "one carry segment per thread");
 struct SegDigits {
   int64_t d[2 * SEG_ELEMS];
@@ -218,6 +220,38 @@ __device__ __forceinline__ int twiddle_span(const Step& st, int lam) { return (1
 
 template <int B, class G61, class G31, class Src, class Dst>
 __device__ __forceinline__ void gather2(e61* L61, e31* L31, G61 g61, G31 g31, int count, Src src, Dst dst);
+
+// A flat range of count <= blockDim.x twiddles (the tables of a pass that
+// starts at level 0, or a centre record's LDS part), one entry per thread:
+// load() issues the global loads into registers, store() writes LDS later,
+// so that their latency overlaps the kernel's data loads instead of adding
+// to them.
+struct FlatTwiddles {
+  uint64_t a = 0, b = 0;
+  uint32_t c = 0, d = 0;
+  __device__ __forceinline__ void load(const e61* __restrict__ g61, const e31* __restrict__ g31, int count) {
+    if (count <= 0) return;
+    const int i = min((int)threadIdx.x, count - 1);
+    a = g61[i].a;
+    b = g61[i].b;
+    c = g31[i].a;
+    d = g31[i].b;
+  }
+  __device__ __forceinline__ void store(e61* T61, e31* T31, int count) const {
+    if ((int)threadIdx.x < count) {
+      T61[threadIdx.x] = e61{a, b};
+      T31[threadIdx.x] = e31{c, d};
+    }
+  }
+};
+// The flat range of an outer pass (levels 0..A-1): its steps' tables lie back
+// to back, 2^A - 1 <= 511 entries, the layout stage_pass_twiddles gives.
+__device__ __forceinline__ int outer_twiddles(FlatTwiddles& ft, const Pass& ps, const e61* __restrict__ tw61,
+                                              const e31* __restrict__ tw31) {
+  const int n = (1 << ps.A) - 1;
+  ft.load(tw61 + ps.steps[0].twoff, tw31 + ps.steps[0].twoff, n);
+  return n;
+}
 
 __device__ __forceinline__ int stage_pass_twiddles(e61* T61, e31* T31, const Pass& ps, int nb, const uint32_t* j0,
                                                    const e61* __restrict__ tw61, const e31* __restrict__ tw31) {
@@ -887,11 +921,11 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
   const uint32_t col0 = cg << logW;
-  const uint32_t j0[1] = {0};
   // each thread takes one carry segment: its digits straight from the
   // strip-major digit buffer (64 contiguous bytes), the carry entering its
   // strip row if it is the row's first segment, then the weights into LDS
-  stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
+  FlatTwiddles ft;
+  const int ntw = outer_twiddles(ft, ps, tw61, tw31);
   {
     const int subs = W / SEG_ELEMS;
     const int it = threadIdx.x;
@@ -905,6 +939,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
       seg_weight(sd, L61, L31, idx0, 2u * e0, D);
     }
   }
+  ft.store(T61, T31, ntw);
   __syncthreads();
   // the last step (not level 0 when there are two or more) writes global
   // memory directly
@@ -941,11 +976,12 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t cg = xcd_strip(blockIdx.x, (1u << logS) >> logW);
   const uint32_t col0 = cg << logW;
-  const uint32_t j0[1] = {0};
+  FlatTwiddles ft;
+  const int ntw = outer_twiddles(ft, ps, tw61, tw31);
   gather2<4>(L61, L31, z61, z31, R * W,
              [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); },
              [](int i) { return PADX(i); });
-  stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
+  ft.store(T61, T31, ntw);
   __syncthreads();
   lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31);
   // thread `it` owns carry segment it: normalise into registers, take the
@@ -1032,7 +1068,6 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
   const uint32_t col0 = cg << logW;
-  const uint32_t j0[1] = {0};
   if (threadIdx.x == 0) epoch = __hip_atomic_load(sync + cg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
   auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
   // the inverse pass's first step (the forward's last) reads global memory
@@ -1041,9 +1076,11 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   const bool edges = ns >= 2;
   const Step sl = ps.steps[ns - 1];
   auto twl = [&](int, int blk) { return sl.twoff + (uint32_t)blk * (uint32_t)(sl.radix - 1); };
+  FlatTwiddles ft;
+  const int ntw = outer_twiddles(ft, ps, tw61, tw31);
   if (edges) edge_pass_step<MP_THREADS, false, true>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
   else gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
-  stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
+  ft.store(T61, T31, ntw);
   __syncthreads();
   // inverse outer pass, unweight + CRT, segment carries (as k_outer_inv);
   // thread `it` owns carry segment it and keeps its digits in registers
@@ -1162,11 +1199,15 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   auto tw0 = [&](int b, int blk) {
     return (uint32_t)(b * twiddle_span(s0, s0.ell - ps.ell0) + blk * (s0.radix - 1));
   };
-  if (edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
-  else gather2<PASS_MAX_ELEMS / NT>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
+  // the LDS part of the record (nE <= CENTRE_EARLY_TW <= NT entries) is
+  // loaded before the data and written to LDS after it
   int nE = 0;
   for (int si = 0; si + 1 < ns; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
-  gather2<1>(T61, T31, r61, r31, nE, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+  FlatTwiddles ft;
+  ft.load(r61, r31, nE);
+  if (edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
+  else gather2<PASS_MAX_ELEMS / NT>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
+  ft.store(T61, T31, nE);
   // register twiddles: slot s of thread t at record entry nE + s * RT + t,
   // RT threads per field (coalesced); M31 lanes keep their e31 packed in .a
   constexpr CentreSlots CS = centre_slots(NT);
