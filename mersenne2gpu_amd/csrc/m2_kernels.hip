@@ -1203,29 +1203,37 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   // loaded before the data and written to LDS after it
   int nE = 0;
   for (int si = 0; si + 1 < ns; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
+  // register twiddles: slot s of thread t at record entry nE + s * RT + t,
+  // RT threads per field (coalesced); M31 lanes keep their e31 packed in .a.
+  // The 1024-thread centre issues these loads first, so that they overlap the
+  // data loads (its 5 slots fit beside the first step's registers); the
+  // 512-thread centre's 11 slots would push it past 128 VGPRs, which two
+  // workgroups per CU need, so it issues them after the first step.
+  constexpr CentreSlots CS = centre_slots(NT);
+  constexpr int NSLOT = centre_tw_slots<NT>();
+  const Lanes<true> ln;
+  e61 tw[NSLOT];
+  auto load_reg_twiddles = [&]() {
+    if (ln.f61) {
+      const M2_GLOBAL e61* src = gptr(r61 + nE) + ln.tid;
+#pragma unroll
+      for (int k = 0; k < NSLOT; ++k) tw[k] = gld(src + k * CS.role_threads);
+    } else {
+      const M2_GLOBAL e31* src = gptr(r31 + nE) + ln.tid;
+#pragma unroll
+      for (int k = 0; k < NSLOT; ++k) {
+        const e31 w = gld(src + k * CS.role_threads);
+        tw[k] = e61{(uint64_t)w.a | ((uint64_t)w.b << 32), 0};
+      }
+    }
+  };
+  if constexpr (NT == MP_THREADS_WIDE) load_reg_twiddles();
   FlatTwiddles ft;
   ft.load(r61, r31, nE);
   if (edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
   else gather2<PASS_MAX_ELEMS / NT>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
   ft.store(T61, T31, nE);
-  // register twiddles: slot s of thread t at record entry nE + s * RT + t,
-  // RT threads per field (coalesced); M31 lanes keep their e31 packed in .a
-  constexpr CentreSlots CS = centre_slots(NT);
-  constexpr int NSLOT = centre_tw_slots<NT>();
-  const Lanes<true> ln;
-  e61 tw[NSLOT];
-  if (ln.f61) {
-    const M2_GLOBAL e61* src = gptr(r61 + nE) + ln.tid;
-#pragma unroll
-    for (int k = 0; k < NSLOT; ++k) tw[k] = gld(src + k * CS.role_threads);
-  } else {
-    const M2_GLOBAL e31* src = gptr(r31 + nE) + ln.tid;
-#pragma unroll
-    for (int k = 0; k < NSLOT; ++k) {
-      const e31 w = gld(src + k * CS.role_threads);
-      tw[k] = e61{(uint64_t)w.a | ((uint64_t)w.b << 32), 0};
-    }
-  }
+  if constexpr (NT != MP_THREADS_WIDE) load_reg_twiddles();
   __syncthreads();
   lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, edge_in ? 1 : 0, ns - 1);
   centre_last_step<true, NT>(L61, L31, ps, nb, tw);
