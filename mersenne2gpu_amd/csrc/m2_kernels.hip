@@ -1056,7 +1056,6 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   __shared__ e61 T61[OUTER_TW];
   __shared__ e31 T31[OUTER_TW];
   __shared__ Carry128 gs[PASS_MAX_ELEMS / SEG_ELEMS];
-  __shared__ uint32_t epoch;
   const TestDesc td = tests[blockIdx.y];
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
@@ -1068,7 +1067,10 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   const int R = 1 << ps.A, W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
   const uint32_t col0 = cg << logW;
-  if (threadIdx.x == 0) epoch = __hip_atomic_load(sync + cg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+  // this launch's count: every thread reads the strip's counter before thread
+  // 0 advances it (after the first barrier), and uses it only after the
+  // inverse pass, so the load's latency is hidden behind the data loads
+  const uint32_t epoch = __hip_atomic_load(sync + cg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
   auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
   // the inverse pass's first step (the forward's last) reads global memory
   // directly, and the forward's last step writes it directly
