@@ -85,10 +85,12 @@ def test_batch_full_tests_verdicts(gpu_ctx):
             assert b.is_prime(i) == (c["p"] in mersenne_prime_exps), (c["p"], c["mode"])
 
 
-def test_batch_multipass_plan(gpu_ctx):
-    """Several tests sharing the multi-pass launches (grid row per test)."""
+@pytest.mark.parametrize("wg", [0, 1024])
+def test_batch_multipass_plan(gpu_ctx, wg):
+    """Several tests sharing the multi-pass launches (grid row per test), with
+    the 512- and the 1024-thread centre."""
     cases = [c for c in _sweep_cases() if 6 <= c["ln"] <= 17 and (c["p"] >> c["ln"]) >= 12]
-    with m2.Batch(gpu_ctx, [c["p"] for c in cases], [c["mode"] for c in cases], local_stage_cap=16) as b:
+    with m2.Batch(gpu_ctx, [c["p"] for c in cases], [c["mode"] for c in cases], local_stage_cap=16, wg=wg) as b:
         done = 0
         for k in (5, 200):
             b.run(k - done)
