@@ -281,6 +281,21 @@ __device__ __forceinline__ int stage_pass_twiddles(e61* T61, e31* T31, const Pas
   return lo;
 }
 
+// End of a radix step: a workgroup barrier, or, with WAVE (each field's whole
+// transform is one wave's), only an ordering point for the wave's own LDS
+// accesses: a wave's LDS instructions complete in issue order, so its later
+// reads see its earlier writes as long as the compiler keeps them in order.
+template <bool WAVE>
+__device__ __forceinline__ void step_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One radix step over data in LDS. M61 values leave forward steps folded
 // once (<= P61 + 7) and inverse steps reduced (<= P61); twiddles are in the
 // split form of mul_tw. The LDS holds nb blocks of R = 2^logR rows
@@ -313,7 +328,7 @@ struct Lanes {
 // TWIST = false is the step at level 0, whose only block has tau = 1: no
 // twiddle products; the inverse then reduces its M61 outputs itself (the
 // carry step's unweighting rotates 61-bit values).
-template <bool FWD, int RADIX, bool SPLIT, bool TWIST = true>
+template <bool FWD, int RADIX, bool SPLIT, bool TWIST = true, bool WAVE = false>
 __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW, int nb, int lam, const e61* Ts61,
                                          const e31* Ts31) {
   constexpr int LR = RADIX == 8 ? 3 : (RADIX == 4 ? 2 : 1);
@@ -387,13 +402,13 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
       for (int e = 0; e < RADIX; ++e) L31[PADX(base + e * stride)] = x[e];
     }
   }
-  __syncthreads();
+  step_sync<WAVE>();
 }
 
 // All steps of a pass, forward (in order) or inverse (reversed), twiddles in
 // LDS as laid out by stage_pass_twiddles. LEVEL0: the pass may start at level
-// 0, whose step runs without twiddle products.
-template <bool FWD, bool SPLIT, bool LEVEL0 = false>
+// 0, whose step runs without twiddle products. WAVE: see step_sync.
+template <bool FWD, bool SPLIT, bool LEVEL0 = false, bool WAVE = false>
 __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, const Pass& ps, const e61* T61,
                                          const e31* T31, int k0 = 0, int k1 = MAX_STEPS) {
   int offs[MAX_STEPS];
@@ -407,14 +422,16 @@ __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, c
     const Step st = ps.steps[si];
     const int lam = st.ell - ps.ell0;
     if (LEVEL0 && st.ell == 0) {
-      if (st.radix == 8) lds_step<FWD, 8, SPLIT, false>(L61, L31, ps.A, logW, nb, lam, T61, T31);
-      else if (st.radix == 4) lds_step<FWD, 4, SPLIT, false>(L61, L31, ps.A, logW, nb, lam, T61, T31);
-      else lds_step<FWD, 2, SPLIT, false>(L61, L31, ps.A, logW, nb, lam, T61, T31);
+      if (st.radix == 8) lds_step<FWD, 8, SPLIT, false, WAVE>(L61, L31, ps.A, logW, nb, lam, T61, T31);
+      else if (st.radix == 4) lds_step<FWD, 4, SPLIT, false, WAVE>(L61, L31, ps.A, logW, nb, lam, T61, T31);
+      else lds_step<FWD, 2, SPLIT, false, WAVE>(L61, L31, ps.A, logW, nb, lam, T61, T31);
       continue;
     }
-    if (st.radix == 8) lds_step<FWD, 8, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
-    else if (st.radix == 4) lds_step<FWD, 4, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
-    else lds_step<FWD, 2, SPLIT>(L61, L31, ps.A, logW, nb, lam, T61 + offs[si], T31 + offs[si]);
+    const e61* t61 = T61 + offs[si];
+    const e31* t31 = T31 + offs[si];
+    if (st.radix == 8) lds_step<FWD, 8, SPLIT, true, WAVE>(L61, L31, ps.A, logW, nb, lam, t61, t31);
+    else if (st.radix == 4) lds_step<FWD, 4, SPLIT, true, WAVE>(L61, L31, ps.A, logW, nb, lam, t61, t31);
+    else lds_step<FWD, 2, SPLIT, true, WAVE>(L61, L31, ps.A, logW, nb, lam, t61, t31);
   }
 }
 
@@ -424,8 +441,8 @@ __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, c
 // twiddle of pair q (block 0's Cb/2 even positions, then block 1's) is
 // s61(k, q) / s31(k, q), k being the thread's iteration: from LDS, or from
 // the registers the centre keeps for iteration k (k < 4: a thread takes at
-// most four pairs, nself * Cb/2 <= 4 x 256).
-template <bool SPLIT, class S61, class S31>
+// most four pairs, nself * Cb/2 <= 4 x 256). WAVE: see step_sync.
+template <bool SPLIT, bool WAVE = false, class S61, class S31>
 __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int nself, S61 s61, S31 s31) {
   const Lanes<SPLIT> ln;
   const int Cb = 1 << logCb;
@@ -469,7 +486,7 @@ __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int n
       }
     }
   }
-  __syncthreads();
+  step_sync<WAVE>();
 }
 
 // Squaring across two LDS blocks B (at 0) and B2 = 3*2^a - 1 - B (at Cb):
@@ -825,23 +842,33 @@ __device__ __forceinline__ void stage_range(e61* S61, e31* S31, const e61* __res
   }
 }
 
-// Threads of the single-workgroup kernel: 256 per field (M61 and M31 waves
-// side by side, as in the multi-pass kernels).
-constexpr int SMALL_THREADS = 512;
-
 // ---------------------------------------------------------------------------
-// Small plan: one workgroup holds the whole residue (h <= 2^SMALL_LOG_H) and
-// runs `iters` iterations without leaving LDS. Digits are in natural order.
-__global__ void __launch_bounds__(SMALL_THREADS) k_small(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
+// Small plan: one workgroup holds the whole residue (h <= 2^LOGH) and runs
+// `iters` iterations without leaving LDS. Digits are in natural order. NTH
+// threads, half per field (M61 and M31 waves side by side, as in the
+// multi-pass kernels): 512 for h <= 2^SMALL_LOG_H, or, for h <= 2^9, 128 —
+// one wave per field, which then holds its field's whole transform, so the
+// radix steps and the squaring need no workgroup barrier (step_sync), only
+// the carry, which reads both fields, does; its LDS (34 KB) also lets several
+// tests of a batch share a CU.
+constexpr int SMALL_THREADS = 512;
+constexpr int SMALL_WAVE_LOG_H = 9;
+constexpr int SMALL_WAVE_THREADS = 128;
+
+template <int LOGH, int NTH>
+__global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                                const e31* __restrict__ tw31, const e61* __restrict__ sq61,
                                                const e31* __restrict__ sq31, Pass ps) {
-  __shared__ e61 L61[padded(1 << SMALL_LOG_H)];
-  __shared__ e31 L31[padded(1 << SMALL_LOG_H)];
-  __shared__ e61 T61[1 << SMALL_LOG_H];
-  __shared__ e31 T31[1 << SMALL_LOG_H];
-  __shared__ e61 S61[1 << (SMALL_LOG_H - 1)];
-  __shared__ e31 S31[1 << (SMALL_LOG_H - 1)];
-  __shared__ Carry128 gs[(1 << SMALL_LOG_H) / SEG_ELEMS];
+  constexpr bool WAVE = NTH == SMALL_WAVE_THREADS;
+  static_assert((1 << LOGH) / SEG_ELEMS <= NTH, "one carry segment per thread");
+  static_assert(!WAVE || LOGH <= SMALL_WAVE_LOG_H, "one wave per field holds the transform");
+  __shared__ e61 L61[padded(1 << LOGH)];
+  __shared__ e31 L31[padded(1 << LOGH)];
+  __shared__ e61 T61[1 << LOGH];
+  __shared__ e31 T31[1 << LOGH];
+  __shared__ e61 S61[1 << (LOGH - 1)];
+  __shared__ e31 S31[1 << (LOGH - 1)];
+  __shared__ Carry128 gs[(1 << LOGH) / SEG_ELEMS];
   const TestDesc td = tests[blockIdx.x];
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   const DigitGen D = td.dgen;
@@ -862,9 +889,11 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_small(const TestDesc* __restr
   for (uint32_t it = 0; it < td.iters; ++it) {
     if (mine) seg_weight(sd, L61, L31, idx0, j, D);
     __syncthreads();
-    lds_pass<true, true, true>(L61, L31, 0, 1, ps, T61, T31);
-    square_self<true>(L61, L31, logH, 1, [&](int, int q) { return S61[q]; }, [&](int, int q) { return S31[q]; });
-    lds_pass<false, true, true>(L61, L31, 0, 1, ps, T61, T31);
+    lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31);
+    square_self<true, WAVE>(L61, L31, logH, 1, [&](int, int q) { return S61[q]; },
+                            [&](int, int q) { return S31[q]; });
+    lds_pass<false, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31);
+    if (WAVE) __syncthreads();  // the carry reads both fields
     if (mine) {
       const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
       store_carry(gs, s, seg_normalise(L61, L31, idx0, j, D, c0, sd));
@@ -1262,8 +1291,12 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
 // Host-side launchers, used by the engine.
 
 hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
-  hipLaunchKernelGGL(k_small, dim3(ntests), dim3(SMALL_THREADS), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31,
-                     P.outer);
+  if (P.L <= SMALL_WAVE_LOG_H)
+    hipLaunchKernelGGL((k_small<SMALL_WAVE_LOG_H, SMALL_WAVE_THREADS>), dim3(ntests), dim3(SMALL_WAVE_THREADS), 0, st,
+                       tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
+  else
+    hipLaunchKernelGGL((k_small<SMALL_LOG_H, SMALL_THREADS>), dim3(ntests), dim3(SMALL_THREADS), 0, st, tests, P.tw61,
+                       P.tw31, P.sq61, P.sq31, P.outer);
   return hipGetLastError();
 }
 
@@ -1386,8 +1419,12 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
 
 hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
                                hipEvent_t start, hipEvent_t stop) {
-  hipExtLaunchKernelGGL(k_small, dim3(ntests), dim3(SMALL_THREADS), 0, st, start, stop, 0, tests, P.tw61, P.tw31, P.sq61,
-                        P.sq31, P.outer);
+  if (P.L <= SMALL_WAVE_LOG_H)
+    hipExtLaunchKernelGGL((k_small<SMALL_WAVE_LOG_H, SMALL_WAVE_THREADS>), dim3(ntests), dim3(SMALL_WAVE_THREADS), 0, st,
+                          start, stop, 0, tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
+  else
+    hipExtLaunchKernelGGL((k_small<SMALL_LOG_H, SMALL_THREADS>), dim3(ntests), dim3(SMALL_THREADS), 0, st, start, stop,
+                          0, tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
   return hipGetLastError();
 }
 
