@@ -935,8 +935,9 @@ __device__ __forceinline__ uint32_t xcd_strip(uint32_t b, uint32_t n) {
 // workgroup's loads and stores overlap another's arithmetic.
 constexpr int OUTER_TW = 1 << PASS_MAX_LOG_R;
 
-__global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
-                                                   const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
+__global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests,
+                                                          const e61* __restrict__ tw61, const e31* __restrict__ tw31,
+                                                          Pass ps, int logS, int logW) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
@@ -988,8 +989,9 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   }
 }
 
-__global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
-                                                   const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
+__global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __restrict__ tests,
+                                                          const e61* __restrict__ tw61, const e31* __restrict__ tw31,
+                                                          Pass ps, int logS, int logW) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
@@ -1291,9 +1293,10 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
 // Host-side launchers, used by the engine.
 
 hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
-  if (P.L <= SMALL_WAVE_LOG_H)
-    hipLaunchKernelGGL((k_small<SMALL_WAVE_LOG_H, SMALL_WAVE_THREADS>), dim3(ntests), dim3(SMALL_WAVE_THREADS), 0, st,
-                       tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
+  constexpr int WL = SMALL_WAVE_LOG_H, WT = SMALL_WAVE_THREADS;
+  if (P.L <= WL)
+    hipLaunchKernelGGL((k_small<WL, WT>), dim3(ntests), dim3(WT), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31,
+                       P.outer);
   else
     hipLaunchKernelGGL((k_small<SMALL_LOG_H, SMALL_THREADS>), dim3(ntests), dim3(SMALL_THREADS), 0, st, tests, P.tw61,
                        P.tw31, P.sq61, P.sq31, P.outer);
@@ -1384,13 +1387,14 @@ hipError_t launch_run_fused_events(const DevPlan& P, const TestDesc* tests, uint
                                    const char** names) {
   if (P.small || P.has_mid || k == 0) return hipErrorInvalidValue;
   int n = 0;
-  hipError_t e = run_fused(P, tests, k, [&](const char* name, auto kernel, dim3 grid, dim3 block, auto... args) -> hipError_t {
+  auto launch = [&](const char* name, auto kernel, dim3 grid, dim3 block, auto... args) -> hipError_t {
     if (n >= max_events) return hipErrorInvalidValue;
     names[n] = name;
     hipExtLaunchKernelGGL(kernel, grid, block, 0, st, starts[n], stops[n], 0, args...);
     ++n;
     return hipGetLastError();
-  });
+  };
+  hipError_t e = run_fused(P, tests, k, launch);
   *count = n;
   return e;
 }
@@ -1406,22 +1410,24 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
                                    hipEvent_t* starts, hipEvent_t* stops, int max_events, int* count,
                                    const char** names) {
   int k = 0;
-  hipError_t e = iteration(P, tests, ntests, [&](const char* name, auto kernel, dim3 grid, dim3 block, auto... args) -> hipError_t {
+  auto launch = [&](const char* name, auto kernel, dim3 grid, dim3 block, auto... args) -> hipError_t {
     if (k >= max_events) return hipErrorInvalidValue;
     names[k] = name;
     hipExtLaunchKernelGGL(kernel, grid, block, 0, st, starts[k], stops[k], 0, args...);
     ++k;
     return hipGetLastError();
-  });
+  };
+  hipError_t e = iteration(P, tests, ntests, launch);
   *count = k;
   return e;
 }
 
 hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
                                hipEvent_t start, hipEvent_t stop) {
-  if (P.L <= SMALL_WAVE_LOG_H)
-    hipExtLaunchKernelGGL((k_small<SMALL_WAVE_LOG_H, SMALL_WAVE_THREADS>), dim3(ntests), dim3(SMALL_WAVE_THREADS), 0, st,
-                          start, stop, 0, tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
+  constexpr int WL = SMALL_WAVE_LOG_H, WT = SMALL_WAVE_THREADS;
+  if (P.L <= WL)
+    hipExtLaunchKernelGGL((k_small<WL, WT>), dim3(ntests), dim3(WT), 0, st, start, stop, 0, tests, P.tw61, P.tw31,
+                          P.sq61, P.sq31, P.outer);
   else
     hipExtLaunchKernelGGL((k_small<SMALL_LOG_H, SMALL_THREADS>), dim3(ntests), dim3(SMALL_THREADS), 0, st, start, stop,
                           0, tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
