@@ -124,6 +124,22 @@ int m2_test_plan(m2_test *t, char *buf, size_t len);
  * keep the value they were made with). */
 int m2_debug_fused_spin_limit(uint32_t polls, uint32_t *previous);
 
+/* PRP run with a Gerbicz error check (the reference has none; its loop,
+ * mersenne2_gpu.cpp:4811-4853, trusts every iteration). Advances a PRP test by
+ * `iters` iterations in blocks of `block`: after every block the product d of
+ * the block-end residues since the last verified state x_0 is updated, and
+ * every `blocks_per_check` blocks (and after the last full block) the identity
+ * d_t = x_0 d_{t-1}^(2^block) is checked with `block` extra squarings. A
+ * failed check rolls the test back to the last verified state and repeats the
+ * blocks; the iterations after the last full block are run twice from the
+ * verified state and compared. *errors (optional) receives the number of
+ * failed checks; three failures in a row return M2_E_STATE. LL tests are
+ * refused (M2_E_INVAL). The results are those of m2_test_run. */
+int m2_test_run_checked(m2_test *t, uint64_t iters, uint32_t block, uint32_t blocks_per_check, uint64_t *errors);
+/* Test hook: flip one bit of the residue after block `block` (1-based; 0: never)
+ * of the next m2_test_run_checked call, once, so that its check must catch it. */
+int m2_debug_gerbicz_fault(uint64_t block);
+
 /* Checkpoints: the canonical residue with p, mode, iteration and Res64 in a
  * small checksummed file (absent in the reference, which cannot stop and
  * resume a test). Save writes a temporary file and renames it over `path`.

@@ -49,6 +49,8 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
                                         _c.c_char_p, _c.c_size_t]),
     "m2_test_plan": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
     "m2_debug_fused_spin_limit": (_c.c_int, [_c.c_uint32, _c.POINTER(_c.c_uint32)]),
+    "m2_test_run_checked": (_c.c_int, [_P, _c.c_uint64, _c.c_uint32, _c.c_uint32, _c.POINTER(_c.c_uint64)]),
+    "m2_debug_gerbicz_fault": (_c.c_int, [_c.c_uint64]),
     "m2_checkpoint_save": (_c.c_int, [_P, _c.c_char_p]),
     "m2_checkpoint_load": (_c.c_int, [_P, _c.c_char_p]),
     "m2_checkpoint_info": (_c.c_int, [_c.c_char_p, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_int),
@@ -157,6 +159,12 @@ class Context:
         return Test(self, p, mode, graph_iters)
 
 
+def debug_gerbicz_fault(block: int) -> None:
+    """Test hook: flip one residue bit after block `block` (1-based) of the
+    next run_checked call, once (m2_debug_gerbicz_fault)."""
+    _chk(lib().m2_debug_gerbicz_fault(int(block)), "m2_debug_gerbicz_fault")
+
+
 def checkpoint_info(path: str) -> Dict[str, object]:
     """Header of a checkpoint file (m2_checkpoint_info; no device needed)."""
     p, mode, it, r = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_uint64(), ctypes.c_uint64()
@@ -262,6 +270,14 @@ class Test:
 
     def load_checkpoint(self, path: str) -> None:
         _chk(lib().m2_checkpoint_load(self.h, os.fsencode(path)), "m2_checkpoint_load")
+
+    def run_checked(self, iters: int, block: int, blocks_per_check: int = 10) -> int:
+        """Advance a PRP test by `iters` iterations under a Gerbicz check
+        (m2_test_run_checked); returns the number of failed checks."""
+        e = ctypes.c_uint64()
+        _chk(lib().m2_test_run_checked(self.h, int(iters), int(block), int(blocks_per_check), ctypes.byref(e)),
+             "m2_test_run_checked")
+        return e.value
 
     def time(self, iters: int) -> float:
         """Device milliseconds for `iters` iterations (hipEvents)."""

@@ -522,6 +522,17 @@ struct m2_batch {
   std::map<Class*, Group> groups;
 };
 
+namespace m2 {
+int test_error(int code, const std::string& msg) { return fail(code, msg); }
+int test_params(const m2_test* t, m2_ctx** ctx, uint64_t* p, int* mode) {
+  if (!t) return fail(M2_E_INVAL, "null test");
+  *ctx = t->ctx;
+  *p = t->obj->p;
+  *mode = t->obj->mode;
+  return M2_OK;
+}
+}  // namespace m2
+
 extern "C" {
 
 const char* m2_last_error(void) { return g_err.c_str(); }

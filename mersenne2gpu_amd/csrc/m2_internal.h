@@ -9,6 +9,9 @@
 #include "m2_arith.h"
 #include "m2_plan.h"
 
+struct m2_ctx;
+struct m2_test;
+
 namespace m2 {
 
 // One test as the kernels see it. Tests of the same transform length share a
@@ -116,6 +119,12 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
                                    const char** names);
 hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
                                hipEvent_t start, hipEvent_t stop);
+
+// ---- engine internals shared with m2_gerbicz.cpp (m2_api.cpp) ----
+// Records `msg` as the calling thread's last error and returns `code`.
+int test_error(int code, const std::string& msg);
+// Context, exponent and mode of a test.
+int test_params(const ::m2_test* t, ::m2_ctx** ctx, uint64_t* p, int* mode);
 
 // ---- host side (m2_host.cpp) ----
 int transform_log2(uint64_t p);

@@ -2,11 +2,13 @@
 //           [-LS cap] [-N max_iters] [--show-defaults] [--enable-chunk64] [--debug-plan]
 //           [--debug-stage-selftest] [--debug-cppint-every N] [--debug-cppint-max-iter N]
 //           [--res64-at k1,k2,...] [--dump-residue path] [--checkpoint path] [--resume path]
+//           [--gerbicz block]
 //
 // Drop-in for the reference's command line (mersenne2_gpu.cpp:3618-4960): same
 // grammar, defaults, stdout lines and exit codes, on top of the libm2 C ABI. It
-// adds a final "Res64: <hex>" line (the reference prints no residue) and the
-// --res64-at / --dump-residue options.
+// adds a final "Res64: <hex>" line (the reference prints no residue), the
+// --res64-at / --dump-residue options, checkpoints, and a Gerbicz error check
+// for PRP runs (--gerbicz: blocks of that many iterations, a check every ten).
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -30,7 +32,7 @@ const char* USAGE_TAIL =
     " <p> [ll|prp] [report_every] [-d device_id] [-R report_seconds] [-W fft_wg] [-CW carry_wg] [-B carry_pairs] "
     "[-LS local_stage_cap] [-N max_iters] [--show-defaults] [--enable-chunk64] [--debug-plan] "
     "[--debug-stage-selftest] [--debug-cppint-every N] [--debug-cppint-max-iter N] [--res64-at k1,k2,...] "
-    "[--dump-residue path] [--checkpoint path] [--resume path]\n";
+    "[--dump-residue path] [--checkpoint path] [--resume path] [--gerbicz block]\n";
 
 int die(const char* what, int code) {
   std::cerr << what << " failed with error " << code << ": " << m2_last_error() << std::endl;
@@ -61,6 +63,7 @@ int main(int argc, char* argv[]) {
   uint32_t debug_cppint_every = 0, debug_cppint_max_iter = 0;
   std::vector<uint64_t> res64_at;
   std::string dump_path, checkpoint_path, resume_path;
+  uint32_t gerbicz_block = 0;
 
   auto need = [&](int i, const char* msg) -> bool {
     if (i + 1 >= argc) {
@@ -125,6 +128,13 @@ int main(int argc, char* argv[]) {
     } else if (arg == "--resume") {
       if (!need(i, "Missing path after --resume")) return 1;
       resume_path = argv[++i];
+    } else if (arg == "--gerbicz") {
+      if (!need(i, "Missing block length after --gerbicz")) return 1;
+      gerbicz_block = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
+      if (gerbicz_block == 0) {
+        std::cerr << "--gerbicz needs a block length > 0\n";
+        return 1;
+      }
     } else if (!report_every_set) {
       report_every = (uint32_t)std::strtoul(arg.c_str(), nullptr, 10);
       report_every_set = true;
@@ -239,15 +249,27 @@ int main(int argc, char* argv[]) {
   };
   report(start);
   const uint64_t batch = std::max<uint32_t>(1u, report_every ? report_every : 1000u);
+  // the Gerbicz check runs ten blocks per call (one check each) and reports per call
+  const bool gerbicz = gerbicz_block && mode == M2_MODE_PRP;
+  constexpr uint32_t GERBICZ_BLOCKS = 10;
+  uint64_t gerbicz_errors = 0;
   uint64_t iter = start;
   size_t next_cp = 0;
   while (iter < total) {
-    uint64_t chunk_end = std::min<uint64_t>(total, (iter / batch + 1) * batch);
+    uint64_t chunk_end = gerbicz ? std::min<uint64_t>(total, iter + (uint64_t)gerbicz_block * GERBICZ_BLOCKS)
+                                 : std::min<uint64_t>(total, (iter / batch + 1) * batch);
     if (next_cp < res64_at.size() && res64_at[next_cp] > iter && res64_at[next_cp] < chunk_end)
       chunk_end = res64_at[next_cp];
-    rc = m2_test_run(t, chunk_end - iter);
-    if (rc == M2_OK) rc = m2_test_sync(t);
-    if (rc != M2_OK) return die("m2_test_run", rc);
+    if (gerbicz) {
+      uint64_t e = 0;
+      rc = m2_test_run_checked(t, chunk_end - iter, gerbicz_block, GERBICZ_BLOCKS, &e);
+      gerbicz_errors += e;
+      if (rc != M2_OK) return die("m2_test_run_checked", rc);
+    } else {
+      rc = m2_test_run(t, chunk_end - iter);
+      if (rc == M2_OK) rc = m2_test_sync(t);
+      if (rc != M2_OK) return die("m2_test_run", rc);
+    }
     iter = chunk_end;
     while (next_cp < res64_at.size() && res64_at[next_cp] <= iter) {
       if (res64_at[next_cp] == iter) {
@@ -257,7 +279,7 @@ int main(int argc, char* argv[]) {
       }
       ++next_cp;
     }
-    if (iter % batch == 0 || iter == total) {
+    if (iter % batch == 0 || iter == total || gerbicz) {
       report(iter);
       if (!checkpoint_path.empty()) {
         rc = m2_checkpoint_save(t, checkpoint_path.c_str());
@@ -273,6 +295,7 @@ int main(int argc, char* argv[]) {
   if (m2_test_res64(t, &r64) != M2_OK) return die("m2_test_res64", -1);
   const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   const double itps = el > 0 ? (double)(total - start) / el : 0.0;
+  if (gerbicz) std::cout << "gerbicz: block " << gerbicz_block << ", " << gerbicz_errors << " failed checks\n";
   std::cout << "2^" << p << " - 1 is "
             << (prime ? (mode == M2_MODE_LL ? "prime" : "a probable prime") : "composite") << ", elapsed "
             << std::fixed << std::setprecision(2) << el << " s, it/s " << std::setprecision(1) << itps << "\n";

@@ -1,4 +1,4 @@
-"""Builds and runs the C++ host-code test (tests/cpp/test_host_plan.cpp) on the CPU."""
+"""Builds and runs the C++ host-code tests (tests/cpp/) on the CPU."""
 import os
 import subprocess
 
@@ -22,6 +22,17 @@ def test_transform_arithmetic(tmp_path):
     csrc = os.path.join(ROOT, "mersenne2gpu_amd", "csrc")
     exe = str(tmp_path / "test_arith")
     subprocess.run(["g++", "-O2", "-std=c++17", "-I", csrc, os.path.join(ROOT, "tests", "cpp", "test_arith.cpp"),
+                    "-o", exe], check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:]
+
+
+def test_gerbicz_residue_arithmetic(tmp_path):
+    """Addition, negation and division by 4 mod 2^p - 1 used by the Gerbicz
+    check (tests/cpp/test_modn.cpp), against 128-bit integers and identities."""
+    csrc = os.path.join(ROOT, "mersenne2gpu_amd", "csrc")
+    exe = str(tmp_path / "test_modn")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", csrc, os.path.join(ROOT, "tests", "cpp", "test_modn.cpp"),
                     "-o", exe], check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-3000:]
