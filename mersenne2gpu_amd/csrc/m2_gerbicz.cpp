@@ -14,9 +14,7 @@
 // ((d + x)^2 - (d - x)^2) / 4, two single squarings plus additions, a
 // complement and a rotation by 2 bits on the host (4^-1 = 2^(p-2) mod 2^p - 1).
 #include <stdint.h>
-#include <string.h>
 
-#include <memory>
 #include <string>
 #include <vector>
 
@@ -33,12 +31,13 @@ using namespace modn;
 // (1-based; 0: never), once.
 uint64_t g_fault_block = 0;
 
+// Residue transfers and the squarings the check needs, on a scratch test s of
+// the same exponent.
 struct Checked {
-  m2_test* t;
-  m2_test* s = nullptr;  // scratch test for x^(2^k)
+  m2_test* s = nullptr;
   ModN m;
   size_t len;
-  Checked(m2_test* t_, uint32_t p) : t(t_), m(p), len((p + 7) / 8) {}
+  explicit Checked(uint32_t p) : m(p), len((p + 7) / 8) {}
   ~Checked() {
     if (s) m2_test_destroy(s);
   }
@@ -85,7 +84,7 @@ int m2_test_run_checked(m2_test* t, uint64_t iters, uint32_t block, uint32_t blo
   int rc = test_params(t, &ctx, &p, &mode);
   if (rc) return rc;
   if (mode != M2_MODE_PRP) return test_error(M2_E_INVAL, "the Gerbicz check applies to PRP tests");
-  Checked c(t, (uint32_t)p);
+  Checked c((uint32_t)p);
   if ((rc = m2_test_create(ctx, p, M2_MODE_PRP, nullptr, &c.s))) return rc;
   uint64_t start = 0;
   if ((rc = m2_test_iteration(t, &start))) return rc;
