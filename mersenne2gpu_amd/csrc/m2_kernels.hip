@@ -846,13 +846,15 @@ __device__ __forceinline__ void stage_range(e61* S61, e31* S31, const e61* __res
 // Small plan: one workgroup holds the whole residue (h <= 2^LOGH) and runs
 // `iters` iterations without leaving LDS. Digits are in natural order. NTH
 // threads, half per field (M61 and M31 waves side by side, as in the
-// multi-pass kernels): 512 for h <= 2^SMALL_LOG_H, or, for h <= 2^9, 128 —
+// multi-pass kernels): 512 for h <= 2^SMALL_LOG_H, or, for h <= 2^8, 128 —
 // one wave per field, which then holds its field's whole transform, so the
 // radix steps and the squaring need no workgroup barrier (step_sync), only
-// the carry, which reads both fields, does; its LDS (34 KB) also lets several
-// tests of a batch share a CU.
+// the carry, which reads both fields, does; its LDS (17 KB) also lets several
+// tests of a batch share a CU. (At h = 2^9 a lane would take two to four
+// butterflies per step in turn: 20.3 against 13.4 us/iteration with 512
+// threads.)
 constexpr int SMALL_THREADS = 512;
-constexpr int SMALL_WAVE_LOG_H = 9;
+constexpr int SMALL_WAVE_LOG_H = 8;
 constexpr int SMALL_WAVE_THREADS = 128;
 
 template <int LOGH, int NTH>
