@@ -134,7 +134,7 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs one test on one GPU; multi-GPU runs are not part of this build "
+        print("bench.py runs one test on one GPU; it has no multi-GPU mode "
               "(independent tests shard across GPUs with no communication)", file=sys.stderr)
         return 2
 

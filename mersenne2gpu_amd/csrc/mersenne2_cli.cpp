@@ -148,7 +148,7 @@ int main(int argc, char* argv[]) {
   (void)enable_chunk64;  // accepted; this engine has no opt-in chunk64 kernels
 
   if (debug_cppint_every != 0u || debug_cppint_max_iter != 0u) {
-    std::cout << "debug_cppint_backend=disabled (no exact bigint backend in this build)\n";
+    std::cout << "debug_cppint_backend=disabled (no exact bigint backend in this engine)\n";
     std::cout << "debug_cppint disabled at runtime; use --res64-at to compare residues with an exact reference\n";
     debug_cppint_every = debug_cppint_max_iter = 0;
   }

@@ -4,7 +4,7 @@
  * A plain-C, single-threaded CPU restatement of the reference's IBDWT squaring
  * iteration for Lucas-Lehmer and base-3 PRP tests of 2^p - 1 (mersenne2_gpu.cpp).
  * Only tests/, __graft_entry__.smoke() and bench.py (its cpu_baseline leg) may
- * load this library, and only as the checker or as the timed CPU baseline. The
+ * load this library, and only to verify results or as the timed CPU baseline. The
  * product (mersenne2gpu_amd/) never links, loads or calls it.
  *
  * Every function cites the reference lines it restates. How each one is pinned

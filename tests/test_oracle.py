@@ -1,4 +1,4 @@
-"""Pins the oracle (oracle/m2_oracle.c) before it is trusted as a checker.
+"""Pins the oracle (oracle/m2_oracle.c) before results are verified against it.
 
 Each oracle function is checked against something that does not restate its
 own formula: Python integers, closed forms, brute-force transforms, the
