@@ -108,13 +108,24 @@ int m2_test_run_checked(m2_test* t, uint64_t iters, uint32_t block, uint32_t blo
     nb = 0;
     return r;
   };
+  // The next block is enqueued on the test before the product and the check
+  // of the block just read, so that their host work and the scratch test's
+  // squarings (its own stream) overlap the test's own GPU work; after a failed
+  // check the block in flight is waited for and discarded by the roll-back.
+  bool queued = false;
   for (uint64_t k = vk; k < full_end;) {
     Num x;
-    if ((rc = m2_test_run(t, block)) || (rc = c.read(t, x))) return rc;
+    if (!queued && (rc = m2_test_run(t, block))) return rc;
+    if ((rc = c.read(t, x))) return rc;  // waits for the block
+    queued = false;
     k += block;
     if (++blocks_run == fault) {  // planted error (test hook): one flipped bit
       x[0] ^= 1ull << 5;
       if ((rc = c.write(t, x, k))) return rc;
+    }
+    if (k < full_end) {
+      if ((rc = m2_test_run(t, block))) return rc;
+      queued = true;
     }
     dprev = d;
     if ((rc = c.mul(d, x, d))) return rc;
@@ -130,6 +141,7 @@ int m2_test_run_checked(m2_test* t, uint64_t iters, uint32_t block, uint32_t blo
       nb = 0;
       fails = 0;
     } else {
+      queued = false;  // roll_back's residue write waits for the block in flight first
       if ((rc = roll_back())) return rc;
       k = vk;
     }
