@@ -435,22 +435,38 @@ __device__ __forceinline__ void lds_pass(e61* L61, e31* L31, int logW, int nb, c
   }
 }
 
+// body(k) for k = 0, 1, ... < KMAX until it returns false: unrolled for
+// KMAX <= 2 (so that a register-slot getter indexed by k resolves at compile
+// time), a loop otherwise (code size).
+template <int KMAX, class F>
+__device__ __forceinline__ void for_k(F&& body) {
+  if constexpr (KMAX <= 2) {
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k)
+      if (!body(k)) break;
+  } else {
+#pragma unroll 1
+    for (int k = 0; k < KMAX; ++k)
+      if (!body(k)) break;
+  }
+}
+
 // Squaring of blocks that pair with themselves: block 0 (positions 0..Cb-1,
 // mirror formula, k = 0 special) at LDS offset 0, and, when nself = 2, block 1
 // (positions Cb..2Cb-1, k pairs with 3Cb-1-k) at LDS offset Cb. The centre
 // twiddle of pair q (block 0's Cb/2 even positions, then block 1's) is
 // s61(k, q) / s31(k, q), k being the thread's iteration: from LDS, or from
 // the registers the centre keeps for iteration k (k < 4: a thread takes at
-// most four pairs, nself * Cb/2 <= 4 x 256). WAVE: see step_sync.
-template <bool SPLIT, bool WAVE = false, class S61, class S31>
+// most KMAX pairs, nself * Cb/2 <= KMAX x threads per field). WAVE: see
+// step_sync.
+template <bool SPLIT, bool WAVE = false, int KMAX = 4, class S61, class S31>
 __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int nself, S61 s61, S31 s31) {
   const Lanes<SPLIT> ln;
   const int Cb = 1 << logCb;
   const int half = Cb >> 1;
-#pragma unroll 1
-  for (int k = 0; k < 4; ++k) {
+  for_k<KMAX>([&](int k) {
     const int q = ln.tid + k * ln.n;
-    if (q >= nself * half) break;
+    if (q >= nself * half) return false;
     const int blk = q >= half ? 1 : 0;
     const int j = q - blk * half;
     const int kk = 2 * j;
@@ -485,21 +501,21 @@ __device__ __forceinline__ void square_self(e61* L61, e31* L31, int logCb, int n
         L31[PADX(mkidx)] = d;
       }
     }
-  }
+    return true;
+  });
   step_sync<WAVE>();
 }
 
 // Squaring across two LDS blocks B (at 0) and B2 = 3*2^a - 1 - B (at Cb):
 // position B*Cb + o pairs with B2*Cb + (Cb - 1 - o). Twiddle index ti: block
 // B's even positions (Cb/2) followed by block B2's (Cb/2); s61/s31 as above.
-template <bool SPLIT, class S61, class S31>
+template <bool SPLIT, int KMAX = 4, class S61, class S31>
 __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, S61 s61, S31 s31) {
   const Lanes<SPLIT> ln;
   const int Cb = 1 << logCb;
-#pragma unroll 1
-  for (int k = 0; k < 4; ++k) {
+  for_k<KMAX>([&](int k) {
     const int o = ln.tid + k * ln.n;
-    if (o >= Cb) break;
+    if (o >= Cb) return false;
     int kidx, mkidx, ti;
     if ((o & 1) == 0) {
       kidx = o;
@@ -522,7 +538,8 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, S61 
       L31[PADX(kidx)] = b;
       L31[PADX(mkidx)] = d;
     }
-  }
+    return true;
+  });
   __syncthreads();
 }
 
@@ -1277,8 +1294,8 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
     const e61 w = pick_slot<CS.sq_slots>(tw, CS.step_slots, k);
     return e31{(uint32_t)w.a, (uint32_t)(w.a >> 32)};
   };
-  if (g == 0) square_self<true>(L61, L31, logC, 2, s61, s31);
-  else square_cross<true>(L61, L31, logC, s61, s31);
+  if (g == 0) square_self<true, false, CS.sq_slots>(L61, L31, logC, 2, s61, s31);
+  else square_cross<true, CS.sq_slots>(L61, L31, logC, s61, s31);
   centre_last_step<false, NT>(L61, L31, ps, nb, tw);
   lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31, 1, edges ? ns - 1 : ns);
   if (edges) {
