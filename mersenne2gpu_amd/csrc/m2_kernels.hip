@@ -1098,9 +1098,16 @@ __device__ __forceinline__ i128 carry_take(M2_GLOBAL uint64_t* cf, uint32_t i, u
   }
 }
 
+// CA > 0: the pass is known at compile time to be A = CA levels in radix-8
+// steps over W = 2^CLOGW columns of stride S = 2^CLOGS (the n = 2^20 single
+// test: 9, 2, 10), so its step geometry and index arithmetic are constants;
+// CA = 0 reads it from the arguments.
+template <int CA, int CLOGW, int CLOGS>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __restrict__ tests,
                                                      const e61* __restrict__ tw61, const e31* __restrict__ tw31,
-                                                     Pass ps, int logS, int logW, uint32_t spin_limit) {
+                                                     Pass ps, int logS_arg, int logW_arg, uint32_t spin_limit) {
+  static_assert(CA == 0 || CA == 9, "the constant form covers three radix-8 steps");
+  const int logS = CA ? CLOGS : logS_arg, logW = CA ? CLOGW : logW_arg;
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
@@ -1114,7 +1121,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   uint32_t* sync = (uint32_t*)gsync;  // generic pointer to global memory, for the atomic builtins
   const DigitGen D = td.dgen;
   const int mode = td.mode;
-  const int R = 1 << ps.A, W = 1 << logW;
+  const int A = CA ? CA : ps.A;
+  const int R = 1 << A, W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
   const uint32_t col0 = cg << logW;
   // this launch's count: every thread reads the strip's counter before thread
@@ -1124,20 +1132,28 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
   // the inverse pass's first step (the forward's last) reads global memory
   // directly, and the forward's last step writes it directly
-  const int ns = ps.nsteps;
+  const int ns = CA ? 3 : ps.nsteps;
   const bool edges = ns >= 2;
   const Step sl = ps.steps[ns - 1];
   auto twl = [&](int, int blk) { return sl.twoff + (uint32_t)blk * (uint32_t)(sl.radix - 1); };
   FlatTwiddles ft;
   const int ntw = outer_twiddles(ft, ps, tw61, tw31);
-  if (edges) edge_pass_step<MP_THREADS, false, true>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
+  if (CA) edge_step<MP_THREADS, false, 8, true>(L61, L31, 9, CLOGW, 1, 6, z61, z31, gidx, tw61, tw31, twl);
+  else if (edges)
+    edge_pass_step<MP_THREADS, false, true>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
   else gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
   ft.store(T61, T31, ntw);
   __syncthreads();
   // inverse outer pass, unweight + CRT, segment carries (as k_outer_inv);
   // thread `it` owns carry segment it and keeps its digits in registers
-  // until the weighting
-  lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31, edges ? 1 : 0);
+  // until the weighting. Constant form: steps at relative levels 6 (edge),
+  // 3 and 0 (untwisted); twiddle blocks at 0 (level 0: 7), 7 (level 3: 56).
+  if (CA) {
+    lds_step<false, 8, true>(L61, L31, 9, CLOGW, 1, 3, T61 + 7, T31 + 7);
+    lds_step<false, 8, true, false>(L61, L31, 9, CLOGW, 1, 0, T61, T31);
+  } else {
+    lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31, edges ? 1 : 0);
+  }
   const int subs = W / SEG_ELEMS;
   const int it = threadIdx.x;
   const bool mine = it < R * subs;
@@ -1167,6 +1183,12 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
     seg_weight(sd, L61, L31, idx0, 2u * e0, D);
   }
   __syncthreads();
+  if (CA) {
+    lds_step<true, 8, true, false>(L61, L31, 9, CLOGW, 1, 0, T61, T31);
+    lds_step<true, 8, true>(L61, L31, 9, CLOGW, 1, 3, T61 + 7, T31 + 7);
+    edge_step<MP_THREADS, true, 8, false>(L61, L31, 9, CLOGW, 1, 6, z61, z31, gidx, tw61, tw31, twl);
+    return;
+  }
   lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31, 0, edges ? ns - 1 : ns);
   if (edges) {
     edge_pass_step<MP_THREADS, true, false>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
@@ -1215,11 +1237,16 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
 // square, inverse. There are h / (2C) workgroups. The twiddles of all but the
 // last step are staged in LDS; each thread loads the twiddles of its own
 // last-step butterflies and squaring pairs straight into registers, so the
-// workgroup needs 66 KB of LDS and two fit on a CU.
-template <int NT>
+// workgroup needs 66 KB of LDS and two fit on a CU. CC > 0: the pass is
+// known at compile time to be 2^CC-element blocks in five radix-4 steps with
+// direct loads and stores at its edges (the 1024-thread centre of the n =
+// 2^20 single test: CC = 10), so its step geometry is constant; CC = 0 reads
+// it from the arguments.
+template <int NT, int CC>
 __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
                                                 const e31* __restrict__ ctw31, uint32_t crec, Pass ps,
                                                 int direct_in) {
+  static_assert(CC == 0 || (CC == 10 && NT == MP_THREADS_WIDE), "the constant form is the wide 2^10 centre");
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
   __shared__ e61 T61[CENTRE_EARLY_TW];
@@ -1227,7 +1254,7 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   const TestDesc td = tests[blockIdx.y];
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
-  const int logC = ps.A, C = 1 << logC;
+  const int logC = CC ? CC : ps.A, C = 1 << logC;
   const uint32_t g = blockIdx.x;
   uint32_t B = 0, B2 = 1;
   const int nb = 2;
@@ -1244,17 +1271,22 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   // where no other workgroup's arithmetic overlaps the loads), the first
   // forward step reads it directly too; step 0's twiddles lead the record
   // (block b's span, then block b+1's).
-  const int ns = ps.nsteps;
+  const int ns = CC ? 5 : ps.nsteps;
   const bool edges = ns >= 2;
-  const bool edge_in = edges && direct_in;
+  const bool edge_in = CC ? true : edges && direct_in;
   const Step s0 = ps.steps[0];
   auto tw0 = [&](int b, int blk) {
+    if (CC) return (uint32_t)(b * 3 + blk * 3);  // step 0: relative level 0, radix 4
     return (uint32_t)(b * twiddle_span(s0, s0.ell - ps.ell0) + blk * (s0.radix - 1));
   };
   // the LDS part of the record (nE <= CENTRE_EARLY_TW <= NT entries) is
-  // loaded before the data and written to LDS after it
+  // loaded before the data and written to LDS after it; constant form: the
+  // spans of steps 0..3 (3, 12, 48, 192 per block, two blocks) at offsets 0,
+  // 6, 30 and 126, 510 in all
   int nE = 0;
-  for (int si = 0; si + 1 < ns; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
+  if (CC) nE = 510;
+  else
+    for (int si = 0; si + 1 < ns; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
   // register twiddles: slot s of thread t at record entry nE + s * RT + t,
   // RT threads per field (coalesced); M31 lanes keep their e31 packed in .a.
   // The 1024-thread centre issues these loads first, so that they overlap the
@@ -1282,13 +1314,21 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   if constexpr (NT == MP_THREADS_WIDE) load_reg_twiddles();
   FlatTwiddles ft;
   ft.load(r61, r31, nE);
-  if (edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
+  if (CC) edge_step<NT, true, 4, true>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
+  else if (edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
   else gather2<PASS_MAX_ELEMS / NT>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
   ft.store(T61, T31, nE);
   if constexpr (NT != MP_THREADS_WIDE) load_reg_twiddles();
   __syncthreads();
-  lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, edge_in ? 1 : 0, ns - 1);
-  centre_last_step<true, NT>(L61, L31, ps, nb, tw);
+  if (CC) {
+    lds_step<true, 4, true>(L61, L31, CC, 0, 2, 2, T61 + 6, T31 + 6);
+    lds_step<true, 4, true>(L61, L31, CC, 0, 2, 4, T61 + 30, T31 + 30);
+    lds_step<true, 4, true>(L61, L31, CC, 0, 2, 6, T61 + 126, T31 + 126);
+    reg_step<true, 4, NT>(L61, L31, CC, 2, 8, tw);
+  } else {
+    lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, edge_in ? 1 : 0, ns - 1);
+    centre_last_step<true, NT>(L61, L31, ps, nb, tw);
+  }
   auto s61 = [&](int k, int) { return pick_slot<CS.sq_slots>(tw, CS.step_slots, k); };
   auto s31 = [&](int k, int) {
     const e61 w = pick_slot<CS.sq_slots>(tw, CS.step_slots, k);
@@ -1296,6 +1336,14 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   };
   if (g == 0) square_self<true, false, CS.sq_slots>(L61, L31, logC, 2, s61, s31);
   else square_cross<true, CS.sq_slots>(L61, L31, logC, s61, s31);
+  if (CC) {
+    reg_step<false, 4, NT>(L61, L31, CC, 2, 8, tw);
+    lds_step<false, 4, true>(L61, L31, CC, 0, 2, 6, T61 + 126, T31 + 126);
+    lds_step<false, 4, true>(L61, L31, CC, 0, 2, 4, T61 + 30, T31 + 30);
+    lds_step<false, 4, true>(L61, L31, CC, 0, 2, 2, T61 + 6, T31 + 6);
+    edge_step<NT, false, 4, false>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
+    return;
+  }
   centre_last_step<false, NT>(L61, L31, ps, nb, tw);
   lds_pass<false, true>(L61, L31, 0, nb, ps, T61, T31, 1, edges ? ns - 1 : ns);
   if (edges) {
@@ -1345,7 +1393,7 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
     return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre * nt <= P.cus ? 1 : 0;
-  if ((e = launch("k_centre", k_centre<NT>, dim3(ncentre, nt), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
+  if ((e = launch("k_centre", k_centre<NT, 0>, dim3(ncentre, nt), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
                   direct_in)) != hipSuccess)
     return e;
   if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid, nt), blk, tests, P.tw61, P.tw31, P.mid, P.mid_logS,
@@ -1366,22 +1414,31 @@ hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& l
 template <int NT, class F>
 hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& launch) {
   hipError_t e;
+  // the outer pass of n = 2^20 (levels 0..8 in three radix-8 steps, W = 4,
+  // S = 1024) has a kernel with that geometry built in
+  const bool c20 = P.outer.A == 9 && P.outer.nsteps == 3 && P.outer.steps[0].radix == 8 &&
+                   P.outer.steps[1].radix == 8 && P.outer.steps[2].radix == 8 && P.logW == 2 && P.logS == 10;
   const uint32_t ns = outer_strips(P);
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre <= P.cus ? 1 : 0;
+  // and so has its 1024-thread centre (2^10-element blocks, five radix-4
+  // steps, one workgroup per CU so that the first step loads directly)
+  bool c10 = NT == MP_THREADS_WIDE && P.centre.A == 10 && P.centre.nsteps == 5 && direct_in;
+  for (int i = 0; c10 && i < 5; ++i) c10 = P.centre.steps[i].radix == 4;
+  auto centre = c10 ? k_centre<NT, NT == MP_THREADS_WIDE ? 10 : 0> : k_centre<NT, 0>;
   const dim3 blk(MP_THREADS), cblk(NT);
   if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
       hipSuccess)
     return e;
   for (uint64_t i = 0; i + 1 < k; ++i) {
-    if ((e = launch("k_centre", k_centre<NT>, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
+    if ((e = launch("k_centre", centre, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
                     direct_in)) != hipSuccess)
       return e;
-    if ((e = launch("k_outer_fused", k_outer_fused, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW,
-                    g_fused_spin_limit)) != hipSuccess)
+    if ((e = launch("k_outer_fused", c20 ? k_outer_fused<9, 2, 10> : k_outer_fused<0, 0, 0>, dim3(ns, 1), blk, tests,
+                    P.tw61, P.tw31, P.outer, P.logS, P.logW, g_fused_spin_limit)) != hipSuccess)
       return e;
   }
-  if ((e = launch("k_centre", k_centre<NT>, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
+  if ((e = launch("k_centre", centre, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
                   direct_in)) != hipSuccess)
     return e;
   return launch("k_outer_inv", k_outer_inv, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
