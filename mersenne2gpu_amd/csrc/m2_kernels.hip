@@ -1242,11 +1242,17 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
 // direct loads and stores at its edges (the 1024-thread centre of the n =
 // 2^20 single test: CC = 10), so its step geometry is constant; CC = 0 reads
 // it from the arguments.
+// The 512-thread forms are held to 128 VGPRs: two workgroups per CU.
 template <int NT, int CC>
-__global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == MP_THREADS_NARROW ? 4 : 1))) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
                                                 const e31* __restrict__ ctw31, uint32_t crec, Pass ps,
                                                 int direct_in) {
-  static_assert(CC == 0 || (CC == 10 && NT == MP_THREADS_WIDE), "the constant form is the wide 2^10 centre");
+  // CC = 10: the step geometry of a 2^10-element block compiled in; on 1024
+  // threads five radix-4 steps with direct first loads, on 512 threads radix
+  // 2, 8, 8, 8 steps after a gather into LDS
+  static_assert(CC == 0 || (CC == 10 && (NT == MP_THREADS_WIDE || NT == MP_THREADS_NARROW)),
+                "the constant forms are the 2^10 centres");
+  constexpr bool WIDE = NT == MP_THREADS_WIDE;
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
   __shared__ e61 T61[CENTRE_EARLY_TW];
@@ -1271,20 +1277,21 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   // where no other workgroup's arithmetic overlaps the loads), the first
   // forward step reads it directly too; step 0's twiddles lead the record
   // (block b's span, then block b+1's).
-  const int ns = CC ? 5 : ps.nsteps;
+  const int ns = CC ? (WIDE ? 5 : 4) : ps.nsteps;
   const bool edges = ns >= 2;
-  const bool edge_in = CC ? true : edges && direct_in;
+  const bool edge_in = CC ? WIDE : edges && direct_in;
   const Step s0 = ps.steps[0];
   auto tw0 = [&](int b, int blk) {
-    if (CC) return (uint32_t)(b * 3 + blk * 3);  // step 0: relative level 0, radix 4
+    if (CC) return (uint32_t)(WIDE ? b * 3 + blk * 3 : b + blk);  // step 0: relative level 0, radix 4 or 2
     return (uint32_t)(b * twiddle_span(s0, s0.ell - ps.ell0) + blk * (s0.radix - 1));
   };
   // the LDS part of the record (nE <= CENTRE_EARLY_TW <= NT entries) is
   // loaded before the data and written to LDS after it; constant form: the
   // spans of steps 0..3 (3, 12, 48, 192 per block, two blocks) at offsets 0,
-  // 6, 30 and 126, 510 in all
+  // 6, 30 and 126, 510 in all (1024 threads); steps 0..2 (1, 14, 112) at
+  // offsets 0, 2 and 30, 254 in all (512 threads)
   int nE = 0;
-  if (CC) nE = 510;
+  if (CC) nE = WIDE ? 510 : 254;
   else
     for (int si = 0; si + 1 < ns; ++si) nE += nb * twiddle_span(ps.steps[si], ps.steps[si].ell - ps.ell0);
   // register twiddles: slot s of thread t at record entry nE + s * RT + t,
@@ -1314,17 +1321,22 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   if constexpr (NT == MP_THREADS_WIDE) load_reg_twiddles();
   FlatTwiddles ft;
   ft.load(r61, r31, nE);
-  if (CC) edge_step<NT, true, 4, true>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
-  else if (edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
+  if constexpr (CC != 0 && WIDE) edge_step<NT, true, 4, true>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
+  else if (!CC && edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
   else gather2<PASS_MAX_ELEMS / NT>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
   ft.store(T61, T31, nE);
   if constexpr (NT != MP_THREADS_WIDE) load_reg_twiddles();
   __syncthreads();
-  if (CC) {
+  if constexpr (CC != 0 && WIDE) {
     lds_step<true, 4, true>(L61, L31, CC, 0, 2, 2, T61 + 6, T31 + 6);
     lds_step<true, 4, true>(L61, L31, CC, 0, 2, 4, T61 + 30, T31 + 30);
     lds_step<true, 4, true>(L61, L31, CC, 0, 2, 6, T61 + 126, T31 + 126);
     reg_step<true, 4, NT>(L61, L31, CC, 2, 8, tw);
+  } else if constexpr (CC != 0) {
+    lds_step<true, 2, true>(L61, L31, CC, 0, 2, 0, T61, T31);
+    lds_step<true, 8, true>(L61, L31, CC, 0, 2, 1, T61 + 2, T31 + 2);
+    lds_step<true, 8, true>(L61, L31, CC, 0, 2, 4, T61 + 30, T31 + 30);
+    reg_step<true, 8, NT>(L61, L31, CC, 2, 7, tw);
   } else {
     lds_pass<true, true>(L61, L31, 0, nb, ps, T61, T31, edge_in ? 1 : 0, ns - 1);
     centre_last_step<true, NT>(L61, L31, ps, nb, tw);
@@ -1336,12 +1348,19 @@ __global__ void __launch_bounds__(NT) k_centre(const TestDesc* __restrict__ test
   };
   if (g == 0) square_self<true, false, CS.sq_slots>(L61, L31, logC, 2, s61, s31);
   else square_cross<true, CS.sq_slots>(L61, L31, logC, s61, s31);
-  if (CC) {
+  if constexpr (CC != 0 && WIDE) {
     reg_step<false, 4, NT>(L61, L31, CC, 2, 8, tw);
     lds_step<false, 4, true>(L61, L31, CC, 0, 2, 6, T61 + 126, T31 + 126);
     lds_step<false, 4, true>(L61, L31, CC, 0, 2, 4, T61 + 30, T31 + 30);
     lds_step<false, 4, true>(L61, L31, CC, 0, 2, 2, T61 + 6, T31 + 6);
     edge_step<NT, false, 4, false>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
+    return;
+  }
+  if constexpr (CC != 0 && !WIDE) {
+    reg_step<false, 8, NT>(L61, L31, CC, 2, 7, tw);
+    lds_step<false, 8, true>(L61, L31, CC, 0, 2, 4, T61 + 30, T31 + 30);
+    lds_step<false, 8, true>(L61, L31, CC, 0, 2, 1, T61 + 2, T31 + 2);
+    edge_step<NT, false, 2, false>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
     return;
   }
   centre_last_step<false, NT>(L61, L31, ps, nb, tw);
@@ -1376,6 +1395,24 @@ hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests
 uint32_t g_fused_spin_limit = 1u << 20;
 
 namespace {
+// Whether the centre of P has the step geometry that k_centre<NT, 10> has
+// compiled in: 2^10-element blocks in five radix-4 steps whose first step
+// loads directly (1024 threads), or in radix-2, 8, 8, 8 steps after a gather
+// (512 threads).
+template <int NT>
+bool centre_const(const DevPlan& P, int direct_in) {
+  constexpr bool wide = NT == MP_THREADS_WIDE;
+  static const int radix_w[] = {4, 4, 4, 4, 4}, lam_w[] = {0, 2, 4, 6, 8};
+  static const int radix_n[] = {2, 8, 8, 8}, lam_n[] = {0, 1, 4, 7};
+  const int ns = wide ? 5 : 4;
+  if (P.centre.A != 10 || P.centre.nsteps != ns || (direct_in != 0) != wide) return false;
+  for (int i = 0; i < ns; ++i)
+    if (P.centre.steps[i].radix != (wide ? radix_w : radix_n)[i] ||
+        P.centre.steps[i].ell - P.centre.ell0 != (wide ? lam_w : lam_n)[i])
+      return false;
+  return true;
+}
+
 // Launches the kernels of one iteration of every test in `tests` through
 // launch(name, kernel, grid, block, args...); grid.y indexes the tests. NT is
 // the centre's workgroup size, P.centre_nt.
@@ -1393,8 +1430,8 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
     return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre * nt <= P.cus ? 1 : 0;
-  if ((e = launch("k_centre", k_centre<NT, 0>, dim3(ncentre, nt), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
-                  direct_in)) != hipSuccess)
+  if ((e = launch("k_centre", centre_const<NT>(P, direct_in) ? k_centre<NT, 10> : k_centre<NT, 0>, dim3(ncentre, nt),
+                  cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) != hipSuccess)
     return e;
   if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid, nt), blk, tests, P.tw61, P.tw31, P.mid, P.mid_logS,
                                P.mid_logW)) != hipSuccess)
@@ -1423,9 +1460,7 @@ hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&&
   const int direct_in = ncentre <= P.cus ? 1 : 0;
   // and so has its 1024-thread centre (2^10-element blocks, five radix-4
   // steps, one workgroup per CU so that the first step loads directly)
-  bool c10 = NT == MP_THREADS_WIDE && P.centre.A == 10 && P.centre.nsteps == 5 && direct_in;
-  for (int i = 0; c10 && i < 5; ++i) c10 = P.centre.steps[i].radix == 4;
-  auto centre = c10 ? k_centre<NT, NT == MP_THREADS_WIDE ? 10 : 0> : k_centre<NT, 0>;
+  auto centre = centre_const<NT>(P, direct_in) ? k_centre<NT, 10> : k_centre<NT, 0>;
   const dim3 blk(MP_THREADS), cblk(NT);
   if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
       hipSuccess)
