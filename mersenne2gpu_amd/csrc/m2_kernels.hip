@@ -1202,9 +1202,14 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
 
 // Middle pass: levels [ell0, ell0 + A) with no carry work. Level-ell0 block
 // j (of R * S elements) is split into S/W column groups.
-template <bool FWD>
+// CA > 0: the pass is known at compile time to be A = CA levels in two
+// radix-8 steps over W = 2^CLOGW columns of stride S = 2^CLOGS (n = 2^24: 6,
+// 5, 10), so its step geometry and index arithmetic are constants; CA = 0
+// reads it from the arguments.
+template <bool FWD, int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                              const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
+  static_assert(CA == 0 || CA == 6, "the constant form is two radix-8 steps");
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
@@ -1212,19 +1217,31 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
   const TestDesc td = tests[blockIdx.y];
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
-  const int R = 1 << ps.A, W = 1 << logW;
-  const uint32_t NS = (1u << logS) >> logW;
+  const int A = CA ? CA : ps.A, lw = CA ? CLOGW : logW, ls = CA ? CLOGS : logS;
+  const int R = 1 << A, W = 1 << lw;
+  const uint32_t NS = (1u << ls) >> lw;
   const uint32_t bx = xcd_strip(blockIdx.x, gridDim.x);
   const uint32_t jb = bx / NS, cg = bx - jb * NS;
-  const uint32_t base = (jb << (ps.A + logS)) + (cg << logW);
+  const uint32_t base = (jb << (A + ls)) + (cg << lw);
   const uint32_t j0[1] = {jb};
   // loads and stores go through LDS (direct global loads or stores at the
   // pass edges measured slower here, where two workgroups share a CU)
-  auto gidx = [&](int i) { return base + ((uint32_t)(i >> logW) << logS) + (uint32_t)(i & (W - 1)); };
+  auto gidx = [&](int i) { return base + ((uint32_t)(i >> lw) << ls) + (uint32_t)(i & (W - 1)); };
   gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
-  lds_pass<FWD, true>(L61, L31, logW, 1, ps, T61, T31);
+  if constexpr (CA != 0) {
+    // relative levels 0 and 3; their twiddle spans (7 and 56) at offsets 0 and 7
+    if constexpr (FWD) {
+      lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+      lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 3, T61 + 7, T31 + 7);
+    } else {
+      lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 3, T61 + 7, T31 + 7);
+      lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+    }
+  } else {
+    lds_pass<FWD, true>(L61, L31, logW, 1, ps, T61, T31);
+  }
   for (int i = threadIdx.x; i < R * W; i += blockDim.x) {
     gst(z61 + gidx(i), L61[PADX(i)]);
     gst(z31 + gidx(i), L31[PADX(i)]);
@@ -1238,10 +1255,11 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
 // last step are staged in LDS; each thread loads the twiddles of its own
 // last-step butterflies and squaring pairs straight into registers, so the
 // workgroup needs 66 KB of LDS and two fit on a CU. CC > 0: the pass is
-// known at compile time to be 2^CC-element blocks in five radix-4 steps with
-// direct loads and stores at its edges (the 1024-thread centre of the n =
-// 2^20 single test: CC = 10), so its step geometry is constant; CC = 0 reads
-// it from the arguments.
+// known at compile time to be 2^CC-element blocks, on 1024 threads in five
+// radix-4 steps with direct loads and stores at its edges (the n = 2^20
+// single test), on 512 threads in radix-2, 8, 8, 8 steps (n >= 2^21, and
+// batches), so its step geometry is constant; CC = 0 reads it from the
+// arguments.
 // The 512-thread forms are held to 128 VGPRs: two workgroups per CU.
 template <int NT, int CC>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == MP_THREADS_NARROW ? 4 : 1))) k_centre(const TestDesc* __restrict__ tests, const e61* __restrict__ ctw61,
@@ -1425,16 +1443,21 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
                   P.logW)) != hipSuccess)
     return e;
   const uint32_t nmid = P.has_mid ? (1u << P.outer.A) * ((1u << P.mid_logS) >> P.mid_logW) : 0;
-  if (P.has_mid && (e = launch("k_mid_fwd", k_mid<true>, dim3(nmid, nt), blk, tests, P.tw61, P.tw31, P.mid, P.mid_logS,
-                               P.mid_logW)) != hipSuccess)
+  // the middle pass of n = 2^24 (6 levels in two radix-8 steps, W = 32,
+  // S = 1024) has a kernel with that geometry built in
+  const bool m6 = P.has_mid && P.mid.A == 6 && P.mid.nsteps == 2 && P.mid.steps[0].radix == 8 &&
+                  P.mid.steps[1].radix == 8 && P.mid.steps[0].ell == P.mid.ell0 &&
+                  P.mid.steps[1].ell == P.mid.ell0 + 3 && P.mid_logW == 5 && P.mid_logS == 10;
+  if (P.has_mid && (e = launch("k_mid_fwd", m6 ? k_mid<true, 6, 5, 10> : k_mid<true>, dim3(nmid, nt), blk, tests,
+                               P.tw61, P.tw31, P.mid, P.mid_logS, P.mid_logW)) != hipSuccess)
     return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre * nt <= P.cus ? 1 : 0;
   if ((e = launch("k_centre", centre_const<NT>(P, direct_in) ? k_centre<NT, 10> : k_centre<NT, 0>, dim3(ncentre, nt),
                   cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) != hipSuccess)
     return e;
-  if (P.has_mid && (e = launch("k_mid_inv", k_mid<false>, dim3(nmid, nt), blk, tests, P.tw61, P.tw31, P.mid, P.mid_logS,
-                               P.mid_logW)) != hipSuccess)
+  if (P.has_mid && (e = launch("k_mid_inv", m6 ? k_mid<false, 6, 5, 10> : k_mid<false>, dim3(nmid, nt), blk, tests,
+                               P.tw61, P.tw31, P.mid, P.mid_logS, P.mid_logW)) != hipSuccess)
     return e;
   return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
                 P.logW);
