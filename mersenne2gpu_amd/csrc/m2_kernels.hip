@@ -954,6 +954,11 @@ __device__ __forceinline__ uint32_t xcd_strip(uint32_t b, uint32_t n) {
 // workgroup's loads and stores overlap another's arithmetic.
 constexpr int OUTER_TW = 1 << PASS_MAX_LOG_R;
 
+// CA > 0: the pass is known at compile time to be A = CA levels in radix-2,
+// 8, 8 steps over W = 2^CLOGW columns of stride S = 2^CLOGS (n = 2^24: 7, 4,
+// 16), so its step geometry and index arithmetic are constants; CA = 0 reads
+// it from the arguments.
+template <int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests,
                                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31,
                                                           Pass ps, int logS, int logW) {
@@ -966,8 +971,10 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
   const M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
+  static_assert(CA == 0 || CA == 7, "the constant form is radix 2, 8, 8");
+  if (CA) logW = CLOGW, logS = CLOGS;
   const DigitGen D = td.dgen;
-  const int R = 1 << ps.A, W = 1 << logW;
+  const int R = 1 << (CA ? CA : ps.A), W = 1 << logW;
   const uint32_t NS = (1u << logS) >> logW, cg = xcd_strip(blockIdx.x, NS);
   const uint32_t col0 = cg << logW;
   // each thread takes one carry segment: its digits straight from the
@@ -993,6 +1000,15 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   // the last step (not level 0 when there are two or more) writes global
   // memory directly
   auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
+  if constexpr (CA != 0) {
+    // relative levels 0 (no twiddles), 1 and 4; spans 1, 14, 112 back to back
+    lds_step<true, 2, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+    lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 1, T61 + 1, T31 + 1);
+    const uint32_t off = ps.steps[2].twoff;
+    edge_step<MP_THREADS, true, 8, false>(L61, L31, CA, CLOGW, 1, 4, z61, z31, gidx, tw61, tw31,
+                                          [&](int, int blk) { return off + (uint32_t)blk * 7u; });
+    return;
+  }
   const int ns = ps.nsteps;
   const bool edges = ns >= 2;
   lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31, 0, edges ? ns - 1 : ns);
@@ -1008,6 +1024,11 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   }
 }
 
+// CA > 0: the pass is known at compile time to be A = CA levels in radix-2,
+// 8, 8 steps over W = 2^CLOGW columns of stride S = 2^CLOGS (n = 2^24: 7, 4,
+// 16), so its step geometry and index arithmetic are constants; CA = 0 reads
+// it from the arguments.
+template <int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __restrict__ tests,
                                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31,
                                                           Pass ps, int logS, int logW) {
@@ -1023,7 +1044,9 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
   const DigitGen D = td.dgen;
   const int mode = td.mode;
-  const int R = 1 << ps.A, W = 1 << logW;
+  static_assert(CA == 0 || CA == 7, "the constant form is radix 2, 8, 8");
+  if (CA) logW = CLOGW, logS = CLOGS;
+  const int R = 1 << (CA ? CA : ps.A), W = 1 << logW;
   const uint32_t cg = xcd_strip(blockIdx.x, (1u << logS) >> logW);
   const uint32_t col0 = cg << logW;
   FlatTwiddles ft;
@@ -1033,7 +1056,13 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
              [](int i) { return PADX(i); });
   ft.store(T61, T31, ntw);
   __syncthreads();
-  lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31);
+  if constexpr (CA != 0) {
+    lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 4, T61 + 15, T31 + 15);
+    lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 1, T61 + 1, T31 + 1);
+    lds_step<false, 2, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+  } else {
+    lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31);
+  }
   // thread `it` owns carry segment it: normalise into registers, take the
   // carry of the segment before it in the strip row, write the row's last
   // carry to G; the digits go back through LDS so that the store to the
@@ -1439,8 +1468,13 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
   hipError_t e;
   const uint32_t ngroups_outer = (1u << P.logS) >> P.logW;
   const dim3 blk(MP_THREADS), cblk(NT);
-  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
-                  P.logW)) != hipSuccess)
+  // the outer pass of n = 2^24 (levels 0..6 in radix-2, 8, 8 steps, W = 16,
+  // S = 2^16) has kernels with that geometry built in
+  const bool o7 = P.outer.A == 7 && P.outer.ell0 == 0 && P.outer.nsteps == 3 && P.outer.steps[0].radix == 2 &&
+                  P.outer.steps[1].radix == 8 && P.outer.steps[2].radix == 8 && P.outer.steps[0].ell == 0 &&
+                  P.outer.steps[1].ell == 1 && P.outer.steps[2].ell == 4 && P.logW == 4 && P.logS == 16;
+  if ((e = launch("k_outer_fwd", o7 ? k_outer_fwd<7, 4, 16> : k_outer_fwd<>, dim3(ngroups_outer, nt), blk, tests,
+                  P.tw61, P.tw31, P.outer, P.logS, P.logW)) != hipSuccess)
     return e;
   const uint32_t nmid = P.has_mid ? (1u << P.outer.A) * ((1u << P.mid_logS) >> P.mid_logW) : 0;
   // the middle pass of n = 2^24 (6 levels in two radix-8 steps, W = 32,
@@ -1459,8 +1493,8 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
   if (P.has_mid && (e = launch("k_mid_inv", m6 ? k_mid<false, 6, 5, 10> : k_mid<false>, dim3(nmid, nt), blk, tests,
                                P.tw61, P.tw31, P.mid, P.mid_logS, P.mid_logW)) != hipSuccess)
     return e;
-  return launch("k_outer_inv", k_outer_inv, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
-                P.logW);
+  return launch("k_outer_inv", o7 ? k_outer_inv<7, 4, 16> : k_outer_inv<>, dim3(ngroups_outer, nt), blk, tests,
+                P.tw61, P.tw31, P.outer, P.logS, P.logW);
 }
 template <class F>
 hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& launch) {
@@ -1485,7 +1519,7 @@ hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&&
   // steps, one workgroup per CU so that the first step loads directly)
   auto centre = centre_const<NT>(P, direct_in) ? k_centre<NT, 10> : k_centre<NT, 0>;
   const dim3 blk(MP_THREADS), cblk(NT);
-  if ((e = launch("k_outer_fwd", k_outer_fwd, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
+  if ((e = launch("k_outer_fwd", k_outer_fwd<>, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
       hipSuccess)
     return e;
   for (uint64_t i = 0; i + 1 < k; ++i) {
@@ -1499,7 +1533,7 @@ hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&&
   if ((e = launch("k_centre", centre, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
                   direct_in)) != hipSuccess)
     return e;
-  return launch("k_outer_inv", k_outer_inv, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
+  return launch("k_outer_inv", k_outer_inv<>, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
 }
 template <class F>
 hipError_t run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& launch) {
