@@ -954,10 +954,10 @@ __device__ __forceinline__ uint32_t xcd_strip(uint32_t b, uint32_t n) {
 // workgroup's loads and stores overlap another's arithmetic.
 constexpr int OUTER_TW = 1 << PASS_MAX_LOG_R;
 
-// CA > 0: the pass is known at compile time to be A = CA levels in radix-2,
-// 8, 8 steps over W = 2^CLOGW columns of stride S = 2^CLOGS (n = 2^24: 7, 4,
-// 16), so its step geometry and index arithmetic are constants; CA = 0 reads
-// it from the arguments.
+// CA > 0: the pass is known at compile time to be A = CA levels over W =
+// 2^CLOGW columns of stride S = 2^CLOGS, in radix-2, 8, 8 steps (n = 2^24: 7,
+// 4, 16) or radix-8, 8, 8 steps (n = 2^20: 9, 2, 10), so its step geometry and
+// index arithmetic are constants; CA = 0 reads it from the arguments.
 template <int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests,
                                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31,
@@ -971,7 +971,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
   const M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
-  static_assert(CA == 0 || CA == 7, "the constant form is radix 2, 8, 8");
+  static_assert(CA == 0 || CA == 7 || CA == 9, "the constant forms are radix 2, 8, 8 and 8, 8, 8");
   if (CA) logW = CLOGW, logS = CLOGS;
   const DigitGen D = td.dgen;
   const int R = 1 << (CA ? CA : ps.A), W = 1 << logW;
@@ -1000,12 +1000,20 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   // the last step (not level 0 when there are two or more) writes global
   // memory directly
   auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
-  if constexpr (CA != 0) {
+  if constexpr (CA == 7) {
     // relative levels 0 (no twiddles), 1 and 4; spans 1, 14, 112 back to back
     lds_step<true, 2, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
     lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 1, T61 + 1, T31 + 1);
     const uint32_t off = ps.steps[2].twoff;
     edge_step<MP_THREADS, true, 8, false>(L61, L31, CA, CLOGW, 1, 4, z61, z31, gidx, tw61, tw31,
+                                          [&](int, int blk) { return off + (uint32_t)blk * 7u; });
+    return;
+  } else if constexpr (CA == 9) {
+    // relative levels 0 (no twiddles), 3 and 6; spans 7, 56, 448 back to back
+    lds_step<true, 8, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+    lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 3, T61 + 7, T31 + 7);
+    const uint32_t off = ps.steps[2].twoff;
+    edge_step<MP_THREADS, true, 8, false>(L61, L31, CA, CLOGW, 1, 6, z61, z31, gidx, tw61, tw31,
                                           [&](int, int blk) { return off + (uint32_t)blk * 7u; });
     return;
   }
@@ -1024,10 +1032,10 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   }
 }
 
-// CA > 0: the pass is known at compile time to be A = CA levels in radix-2,
-// 8, 8 steps over W = 2^CLOGW columns of stride S = 2^CLOGS (n = 2^24: 7, 4,
-// 16), so its step geometry and index arithmetic are constants; CA = 0 reads
-// it from the arguments.
+// CA > 0: the pass is known at compile time to be A = CA levels over W =
+// 2^CLOGW columns of stride S = 2^CLOGS, in radix-2, 8, 8 steps (n = 2^24: 7,
+// 4, 16) or radix-8, 8, 8 steps (n = 2^20: 9, 2, 10), so its step geometry and
+// index arithmetic are constants; CA = 0 reads it from the arguments.
 template <int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __restrict__ tests,
                                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31,
@@ -1044,7 +1052,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
   const DigitGen D = td.dgen;
   const int mode = td.mode;
-  static_assert(CA == 0 || CA == 7, "the constant form is radix 2, 8, 8");
+  static_assert(CA == 0 || CA == 7 || CA == 9, "the constant forms are radix 2, 8, 8 and 8, 8, 8");
   if (CA) logW = CLOGW, logS = CLOGS;
   const int R = 1 << (CA ? CA : ps.A), W = 1 << logW;
   const uint32_t cg = xcd_strip(blockIdx.x, (1u << logS) >> logW);
@@ -1056,10 +1064,14 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
              [](int i) { return PADX(i); });
   ft.store(T61, T31, ntw);
   __syncthreads();
-  if constexpr (CA != 0) {
+  if constexpr (CA == 7) {
     lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 4, T61 + 15, T31 + 15);
     lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 1, T61 + 1, T31 + 1);
     lds_step<false, 2, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+  } else if constexpr (CA == 9) {
+    lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 6, T61 + 63, T31 + 63);
+    lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 3, T61 + 7, T31 + 7);
+    lds_step<false, 8, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
   } else {
     lds_pass<false, true, true>(L61, L31, logW, 1, ps, T61, T31);
   }
@@ -1460,6 +1472,20 @@ bool centre_const(const DevPlan& P, int direct_in) {
   return true;
 }
 
+// The A of the outer-pass geometry that k_outer_fwd/k_outer_inv<A, ...> have
+// compiled in (7: radix-2, 8, 8 steps, W = 16, S = 2^16; 9: radix-8 steps,
+// W = 4, S = 2^10) that P's outer pass has, or 0.
+int outer_const(const DevPlan& P) {
+  const Pass& o = P.outer;
+  if (o.ell0 != 0 || o.nsteps != 3 || o.steps[0].ell != 0 || o.steps[1].radix != 8 || o.steps[2].radix != 8)
+    return 0;
+  if (o.A == 7 && o.steps[0].radix == 2 && o.steps[1].ell == 1 && o.steps[2].ell == 4 && P.logW == 4 && P.logS == 16)
+    return 7;
+  if (o.A == 9 && o.steps[0].radix == 8 && o.steps[1].ell == 3 && o.steps[2].ell == 6 && P.logW == 2 && P.logS == 10)
+    return 9;
+  return 0;
+}
+
 // Launches the kernels of one iteration of every test in `tests` through
 // launch(name, kernel, grid, block, args...); grid.y indexes the tests. NT is
 // the centre's workgroup size, P.centre_nt.
@@ -1468,13 +1494,14 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
   hipError_t e;
   const uint32_t ngroups_outer = (1u << P.logS) >> P.logW;
   const dim3 blk(MP_THREADS), cblk(NT);
-  // the outer pass of n = 2^24 (levels 0..6 in radix-2, 8, 8 steps, W = 16,
-  // S = 2^16) has kernels with that geometry built in
-  const bool o7 = P.outer.A == 7 && P.outer.ell0 == 0 && P.outer.nsteps == 3 && P.outer.steps[0].radix == 2 &&
-                  P.outer.steps[1].radix == 8 && P.outer.steps[2].radix == 8 && P.outer.steps[0].ell == 0 &&
-                  P.outer.steps[1].ell == 1 && P.outer.steps[2].ell == 4 && P.logW == 4 && P.logS == 16;
-  if ((e = launch("k_outer_fwd", o7 ? k_outer_fwd<7, 4, 16> : k_outer_fwd<>, dim3(ngroups_outer, nt), blk, tests,
-                  P.tw61, P.tw31, P.outer, P.logS, P.logW)) != hipSuccess)
+  // the outer passes of n = 2^24 (levels 0..6 in radix-2, 8, 8 steps, W = 16,
+  // S = 2^16) and n = 2^20 (levels 0..8 in radix-8 steps, W = 4, S = 2^10)
+  // have kernels with that geometry built in
+  const int oc = outer_const(P);
+  auto outer_fwd = oc == 7 ? k_outer_fwd<7, 4, 16> : oc == 9 ? k_outer_fwd<9, 2, 10> : k_outer_fwd<>;
+  auto outer_inv = oc == 7 ? k_outer_inv<7, 4, 16> : oc == 9 ? k_outer_inv<9, 2, 10> : k_outer_inv<>;
+  if ((e = launch("k_outer_fwd", outer_fwd, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
+                  P.logW)) != hipSuccess)
     return e;
   const uint32_t nmid = P.has_mid ? (1u << P.outer.A) * ((1u << P.mid_logS) >> P.mid_logW) : 0;
   // the middle pass of n = 2^24 (6 levels in two radix-8 steps, W = 32,
@@ -1493,8 +1520,8 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
   if (P.has_mid && (e = launch("k_mid_inv", m6 ? k_mid<false, 6, 5, 10> : k_mid<false>, dim3(nmid, nt), blk, tests,
                                P.tw61, P.tw31, P.mid, P.mid_logS, P.mid_logW)) != hipSuccess)
     return e;
-  return launch("k_outer_inv", o7 ? k_outer_inv<7, 4, 16> : k_outer_inv<>, dim3(ngroups_outer, nt), blk, tests,
-                P.tw61, P.tw31, P.outer, P.logS, P.logW);
+  return launch("k_outer_inv", outer_inv, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
+                P.logW);
 }
 template <class F>
 hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& launch) {
@@ -1508,10 +1535,9 @@ hipError_t iteration(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&& l
 template <int NT, class F>
 hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& launch) {
   hipError_t e;
-  // the outer pass of n = 2^20 (levels 0..8 in three radix-8 steps, W = 4,
-  // S = 1024) has a kernel with that geometry built in
-  const bool c20 = P.outer.A == 9 && P.outer.nsteps == 3 && P.outer.steps[0].radix == 8 &&
-                   P.outer.steps[1].radix == 8 && P.outer.steps[2].radix == 8 && P.logW == 2 && P.logS == 10;
+  // the outer passes of n = 2^20 (levels 0..8 in three radix-8 steps, W = 4,
+  // S = 1024) have kernels with that geometry built in
+  const bool c20 = outer_const(P) == 9;
   const uint32_t ns = outer_strips(P);
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre <= P.cus ? 1 : 0;
@@ -1519,8 +1545,8 @@ hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&&
   // steps, one workgroup per CU so that the first step loads directly)
   auto centre = centre_const<NT>(P, direct_in) ? k_centre<NT, 10> : k_centre<NT, 0>;
   const dim3 blk(MP_THREADS), cblk(NT);
-  if ((e = launch("k_outer_fwd", k_outer_fwd<>, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW)) !=
-      hipSuccess)
+  if ((e = launch("k_outer_fwd", c20 ? k_outer_fwd<9, 2, 10> : k_outer_fwd<>, dim3(ns, 1), blk, tests, P.tw61,
+                  P.tw31, P.outer, P.logS, P.logW)) != hipSuccess)
     return e;
   for (uint64_t i = 0; i + 1 < k; ++i) {
     if ((e = launch("k_centre", centre, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
@@ -1533,7 +1559,8 @@ hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&&
   if ((e = launch("k_centre", centre, dim3(ncentre, 1), cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre,
                   direct_in)) != hipSuccess)
     return e;
-  return launch("k_outer_inv", k_outer_inv<>, dim3(ns, 1), blk, tests, P.tw61, P.tw31, P.outer, P.logS, P.logW);
+  return launch("k_outer_inv", c20 ? k_outer_inv<9, 2, 10> : k_outer_inv<>, dim3(ns, 1), blk, tests, P.tw61, P.tw31,
+                P.outer, P.logS, P.logW);
 }
 template <class F>
 hipError_t run_fused(const DevPlan& P, const TestDesc* tests, uint64_t k, F&& launch) {
