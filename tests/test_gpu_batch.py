@@ -99,6 +99,23 @@ def test_batch_multipass_plan(gpu_ctx, wg):
                 assert b.res64(i) == c["checkpoints"][str(k)]["res64"], (c["p"], c["mode"], k)
 
 
+def test_batch_n2p20_checkpoints(gpu_ctx):
+    """Batched tests at n = 2^20 run the separate outer passes and the
+    512-thread centre with that length's step geometry compiled in (a single
+    test runs the fused pass and the 1024-thread centre instead): every test
+    of the batch against the checkpoints of the same exponent."""
+    c = load_fixture("c2.json")["cases"][0]
+    ks = [k for k in sorted(int(k) for k in c["checkpoints"]) if k <= 1000]
+    print("batch of 3, p=%d %s ks=%s" % (c["p"], c["mode"], ks), flush=True)
+    with m2.Batch(gpu_ctx, [c["p"]] * 3, c["mode"]) as b:
+        done = 0
+        for k in ks:
+            b.run(k - done)
+            done = k
+            for i in range(3):
+                assert b.res64(i) == c["checkpoints"][str(k)]["res64"], (i, k)
+
+
 def test_batch_rejects_bad_input(gpu_ctx):
     with pytest.raises(m2.M2Error):
         m2.Batch(gpu_ctx, [127, 2], ["prp", "prp"])
