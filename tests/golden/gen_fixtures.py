@@ -6,6 +6,7 @@ Python integers), never from the transform code under test. Usage:
     python tests/golden/gen_fixtures.py c2                   # a few minutes (GMP)
     python tests/golden/gen_fixtures.py c3                   # ~20-30 minutes (GMP)
     python tests/golden/gen_fixtures.py c5                   # a few minutes (GMP)
+    python tests/golden/gen_fixtures.py mid                  # a few minutes (GMP)
 """
 from __future__ import annotations
 
@@ -99,6 +100,17 @@ def gen_c3():
     p = W.draw_uniform(1, *W.C3_RANGE)
     write("c3.json", {"desc": "n=2^24 LL, seed 1: Res64 and SHA-256 of the residue at 100/1000",
                       "seed": 1, "cases": [case(p, "ll", [1, 2, 10, 100, 1000])]})
+
+
+def gen_mid():
+    # one exponent per transform length 2^18 .. 2^23 (the middle of its range),
+    # PRP and LL alternately
+    cases = []
+    for ln in range(18, 24):
+        lo, hi = W.exponent_range(ln)
+        p = W.prime_near((lo + hi) // 2, lo, hi)
+        cases.append(case(p, "prp" if ln % 2 == 0 else "ll", [2, 10, 50]))
+    write("mid.json", {"desc": "one exponent per length n = 2^18..2^23: Res64 at 2/10/50", "cases": cases})
 
 
 def gen_c4():

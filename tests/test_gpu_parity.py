@@ -84,6 +84,15 @@ def test_c2_checkpoints_n2p20(gpu_ctx):
         _check(gpu_ctx, c)
 
 
+def test_lengths_2p18_to_2p23_checkpoints(gpu_ctx):
+    """One exponent per length between the n = 2^20 and n = 2^24 workloads
+    (tests/golden/mid.json, exact GMP arithmetic): their plans mix outer and
+    middle passes that take the step geometry as arguments with the centres
+    that have it compiled in."""
+    for c in load_fixture("mid.json")["cases"]:
+        _check(gpu_ctx, c)
+
+
 def test_c2_centre_workgroup_sizes(gpu_ctx):
     """The 512-thread (radix-8) and 1024-thread (radix-4) centre give the
     same residues; the plan text names the size in use, and a single test
