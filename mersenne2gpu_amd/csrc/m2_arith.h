@@ -327,6 +327,33 @@ M2_HD void dft8_inv_lazy(e61* v) {
   v[7] = {fold61(y3a + (P61x4 - y7a)), fold61(y3b + (P61x4 - y7b))};
 }
 
+// ---- 92-bit CRT (Garner) ----
+//
+// x from its residues s = x mod P61 and t = x mod P31 (either may be in its
+// second representation of 0, P61 or P31): with P31^-1 = 2^31 + 1 (mod P61),
+// u = (s - t)(2^31 + 1) mod P61 and x = u P31 + t in [0, M), M = P61 P31
+// (the reference's unsigned reading, mersenne2_gpu.cpp:2018-2030).
+//
+// The squaring's convolution terms are non-negative as long as every digit is
+// (PRP always; LL with the -2 applied as described at the carry), and with
+// digits d_j in [0, 2^width_j] each term is at most 2^(ln+2q+3) - 2^(2q+2)
+// <= 2^92 - 2^64 for ln <= 27 (q = p >> ln, ln + 2q + 2 <= 91): every pair
+// d_i d_j 2^e of the weighted cyclic convolution has width_i, width_j <= q + 1
+// and e in {0, 1}, and the two pairs with i = 0 or j = 0 have e = 0. Such a
+// term has u = floor(x / P31) <= P61 - 7 2^30 + 1. Values with
+// u >= P61 - 2^30 are read as x - M = (u - P61) P31 + t: negative terms down
+// to -2^60, whichever representation of 0 a residue has (only the smallest
+// exponents, whose first digit is 1 bit wide, make any). `add` (< 2^63) is added to the result: the carry step
+// adds each LL digit's all-ones value there.
+constexpr uint64_t CRT_NEG_U = P61 - (uint64_t(1) << 30);
+M2_HD __int128 crt61_31(uint64_t s, uint32_t t, uint64_t add = 0) {
+  uint64_t u = sub61(s, t);
+  u = add61(u, rot61(u, 31));  // * (2^31 + 1)
+  const int64_t v = (int64_t)u - ((u >= CRT_NEG_U) ? (int64_t)P61 : 0);
+  // v 2^31 - v + t (+ add) as a 128-bit two's-complement integer
+  return ((__int128)v << 31) - (__int128)v + (__int128)((uint64_t)t + add);
+}
+
 // Real-data pairing and squaring of one pair of spectrum values (the
 // reference's square_half, mersenne2_gpu.cpp:856-870): with
 // u0 = z_k + conj(z_mk), u1 = z_k - conj(z_mk), v0 = u0^2 - u1^2 tw,

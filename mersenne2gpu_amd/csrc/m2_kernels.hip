@@ -15,8 +15,9 @@
 // ripples it upward, but its top digit keeps whatever is left instead of
 // passing it on. The represented value is unchanged, every digit but a
 // segment's top one is < 2^width, and the top one is <= 2^width as long as
-// the bits below it exceed the bits of g (8 digits of >= 12 bits each).
-// Convolution terms then stay below n * 2^(2q+2) <= 2^91 < M61*M31.
+// the bits below it exceed the bits of g (8 digits of >= 12 bits each). No
+// digit is negative (LL's -2 included, see seg_normalise), so convolution
+// terms stay in [0, 2^(ln+2q+3) - 2^(2q+2)], below M61*M31 (crt61_31).
 //
 // Twiddles: a workgroup copies the twiddle blocks of its interior radix steps
 // into LDS with coalesced loads at the start. The centre keeps the twiddles of
@@ -63,23 +64,18 @@ __device__ __forceinline__ uint32_t digit31(int64_t d) {
 }
 
 // Unweight (x 2^-(w + ln + 1), which also removes 1/h and the centre's 4) and
-// Garner CRT: x = t + M31 * ((s - t) * (2^31 + 1) mod M61), then signed.
-__device__ __forceinline__ i128 crt_digit(uint64_t s, uint32_t t, uint32_t w61, uint32_t w31, const DigitGen& D) {
+// Garner CRT (crt61_31: x in [0, M) read unsigned, a narrow band below M as
+// x - M), plus `ones` (the digit's all-ones value 2^width - 1 in LL tests,
+// else 0; see seg_normalise).
+__device__ __forceinline__ i128 crt_digit(uint64_t s, uint32_t t, uint32_t w61, uint32_t w31, const DigitGen& D,
+                                          uint64_t ones) {
   uint32_t a61 = w61 + D.k61;
   a61 -= (a61 >= 61u) ? 61u : 0u;
   uint32_t a31 = w31 + D.k31;
   a31 -= (a31 >= 31u) ? 31u : 0u;
   s = rot61(s, 61u - a61);  // a rotation by 61 (31) is the identity: no select for a = 0
   t = rot31(t, 31u - a31);
-  uint64_t u = sub61(s, t);
-  u = add61(u, rot61(u, 31));
-  // x = u (2^31 - 1) + t in [0, M), M = P61 P31 = P61 (2^31 - 1). Values at or
-  // above about 3 2^90 stand for x - M = (u - P61)(2^31 - 1) + t. Deciding on
-  // u >= 3 2^59 instead of x >= 3 2^90 differs only for x within 3 2^59 below
-  // 3 2^90, where no value lies: convolution terms are below 2^91 and the
-  // negative ones (from LL's -2) are tiny.
-  const int64_t v = (int64_t)u - ((u >= (3ull << 59)) ? (int64_t)P61 : 0);
-  return ((i128)v << 31) - (i128)v + (i128)t;
+  return crt61_31(s, t, ones);
 }
 
 // acc >> w (arithmetic) for 0 <= w < 64, as two 64-bit shifts of the words.
@@ -140,20 +136,27 @@ struct SegDigits {
   int64_t d[2 * SEG_ELEMS];
 };
 
-// Unweight + CRT + zero-carry-in normalisation of the segment's spectrum at
-// LDS index idx0 into registers; returns the carry-out.
+// Unweight + CRT + normalisation of the segment's spectrum at LDS index idx0
+// into registers, with carry-in `carry`; returns the carry-out.
+//
+// LL tests subtract 2 as carry-in -2 at digit 0, and (ll_ones = all ones)
+// add the all-ones digit vector, whose value is 2^p - 1 = 0 (mod 2^p - 1), to
+// the terms: every digit's sum is then the term plus 2^width - 1 and digit
+// 0's is the term plus 2^width - 3 >= 0 (width >= 2), so no sum, digit or
+// carry is ever negative and the next squaring's terms stay in the range the
+// CRT reads (crt61_31). PRP tests pass ll_ones = 0 and carry-in 0.
 __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, int idx0, uint32_t j0,
-                                              const DigitGen& D, i128 carry, SegDigits& s) {
+                                              const DigitGen& D, i128 carry, uint64_t ll_ones, SegDigits& s) {
   DWalk w = dwalk_at(D, j0);
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
     const e61 v = L61[PADX(idx0 + i)];
     const e31 y = L31[PADX(idx0 + i)];
     const uint32_t wa = dwalk_width(D, w);
-    i128 xa = crt_digit(v.a, y.a, dwalk_w61(w), dwalk_w31(w), D);
+    i128 xa = crt_digit(v.a, y.a, dwalk_w61(w), dwalk_w31(w), D, ((1ull << wa) - 1ull) & ll_ones);
     dwalk_next(D, w);
     const uint32_t wb = dwalk_width(D, w);
-    i128 xb = crt_digit(v.b, y.b, dwalk_w61(w), dwalk_w31(w), D);
+    i128 xb = crt_digit(v.b, y.b, dwalk_w61(w), dwalk_w31(w), D, ((1ull << wb) - 1ull) & ll_ones);
     dwalk_next(D, w);
     s.d[2 * i] = norm_digit(xa + carry, wa, carry);
     s.d[2 * i + 1] = norm_digit(xb + carry, wb, carry);
@@ -915,7 +918,7 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
     if (WAVE) __syncthreads();  // the carry reads both fields
     if (mine) {
       const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
-      store_carry(gs, s, seg_normalise(L61, L31, idx0, j, D, c0, sd));
+      store_carry(gs, s, seg_normalise(L61, L31, idx0, j, D, c0, td.mode == 0 ? ~0ull : 0ull, sd));
     }
     __syncthreads();
     if (mine) seg_ripple(sd, j, D, load_carry(gs, (s + nseg - 1) % nseg));
@@ -1089,7 +1092,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   SegDigits sd;
   if (mine) {
     const i128 c0 = (mode == 0 && e0 == 0) ? (i128)(-2) : (i128)0;
-    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0, sd));
+    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0, mode == 0 ? ~0ull : 0ull, sd));
   }
   __syncthreads();
   if (mine) {
@@ -1204,7 +1207,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   SegDigits sd;
   if (mine) {
     const i128 c0 = (mode == 0 && e0 == 0) ? (i128)(-2) : (i128)0;
-    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0, sd));
+    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0, mode == 0 ? ~0ull : 0ull, sd));
   }
   __syncthreads();
   if (threadIdx.x == 0) {

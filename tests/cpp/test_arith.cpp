@@ -154,6 +154,54 @@ int main() {
     const e31 x0 = sub(sqr(w0), mul(sqr(w1), t3)), x1 = mul(w0, add(w1, w1));
     CHECK(eq31(p, add(x0, x1)) && eq31(q, conj(sub(x0, x1))), "square_pair 31");
   }
+  // 92-bit CRT: every x in [-2^60, 2^92 - 2^64] comes back from
+  // (x mod P61, x mod P31), with either residue in its second representation
+  // of 0 where it is 0; the edges of the old 3 2^90 rule and of the band
+  // below M are taken explicitly.
+  {
+    typedef __int128 i128;
+    const i128 M = (i128)P61 * P31;
+    const i128 POS_MAX = ((i128)1 << 92) - ((i128)1 << 64);
+    const i128 NEG_MIN = -((i128)1 << 60);
+    auto res61 = [](i128 x) { i128 r = x % (i128)P61; return (uint64_t)(r < 0 ? r + P61 : r); };
+    auto res31 = [](i128 x) { i128 r = x % (i128)P31; return (uint32_t)(r < 0 ? r + P31 : r); };
+    auto check = [&](i128 x, uint64_t add) {
+      uint64_t s = res61(x);
+      uint32_t t = res31(x);
+      for (int alt = 0; alt < 4; ++alt) {
+        const uint64_t sa = (alt & 1) && s == 0 ? P61 : s;
+        const uint32_t ta = (alt & 2) && t == 0 ? P31 : t;
+        const i128 got = crt61_31(sa, ta, add);
+        CHECK(got == x + (i128)add, "crt61_31 x=%016llx%016llx alt=%d", (unsigned long long)(uint64_t)(x >> 64),
+              (unsigned long long)(uint64_t)x, alt);
+      }
+    };
+    const i128 edges[] = {0, 1, 2, P31 - 1, P31, (i128)P61, (i128)3 << 90, ((i128)3 << 90) - 1, ((i128)3 << 90) + 1,
+                          ((i128)3 << 90) + ((i128)3 << 59), ((i128)1 << 91), ((i128)1 << 92) - ((i128)1 << 66),
+                          POS_MAX, POS_MAX - 1, -1, -2, -(i128)P31, -((i128)1 << 40), NEG_MIN, NEG_MIN + 1};
+    for (const i128 x : edges) check(x, 0);
+    for (int i = 0; i < N; ++i) {
+      const int band = (int)(rng() % 4);
+      i128 x;
+      if (band == 0) x = (i128)((((u128)rng() << 64) | rng()) % (u128)(POS_MAX + 1));  // any positive term
+      else if (band == 1) x = ((i128)3 << 90) + (i128)((((u128)rng() << 64) | rng()) % (u128)(POS_MAX - ((i128)3 << 90) + 1));  // the old rule's band
+      else if (band == 2) x = POS_MAX - (i128)(rng() % (1ull << 40));                  // just below the bound
+      else x = -(i128)(1 + rng() % (uint64_t)(-(NEG_MIN)));                            // negative terms
+      check(x, i % 5 == 0 ? (rng() % (1ull << 40)) : 0);
+    }
+    // the reading the band replaced (negative from u >= 3 2^59) misreads the
+    // terms this check covers; the check above must see the difference
+    auto old_rule = [](uint64_t s, uint32_t t) {
+      uint64_t u = sub61(s, t);
+      u = add61(u, rot61(u, 31));
+      const int64_t v = (int64_t)u - ((u >= (3ull << 59)) ? (int64_t)P61 : 0);
+      return ((i128)v << 31) - (i128)v + (i128)t;
+    };
+    const i128 hi_term = ((i128)3 << 90) + 12345;
+    CHECK(old_rule(res61(hi_term), res31(hi_term)) == hi_term - M, "old rule reads 3 2^90 + 12345 as negative");
+    CHECK(crt61_31(res61(hi_term), res31(hi_term)) == hi_term, "crt61_31 reads 3 2^90 + 12345 as positive");
+    CHECK(crt61_31(res61(M - 1), res31(M - 1)) == -1, "M - 1 is -1");
+  }
   printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
   return fails ? 1 : 0;
 }
