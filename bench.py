@@ -12,12 +12,24 @@ Prints one JSON line. Extra fields:
   roofline      dominant kernel's algorithmic HBM bytes per launch / its mean
                 duration (HIP events on the test's stream) against 8 TB/s;
                 traffic = measured HBM bytes per launch from the committed
-                rocprofv3 counter summary, when present;
+                rocprofv3 counter summary named in traffic_source;
   int_roofline  the integer co-bound: algorithmic base products per iteration
                 (49.5 h per field) per second, and the dominant kernel's VALU
                 busy fraction per SIMD from the committed counter summary;
-  cpu_baseline  the single-threaded CPU restatement of the reference's
-                algorithm (oracle/) timed on a bounded sample on this host.
+  batched       the metric's batched half: 512 PRP tests (n = 2^8..2^16,
+                seed 1) for 2000 iterations each in one m2_batch, tests/s
+                against the plain form (one m2_test after another) measured
+                on a seeded sample; every Res64 checked;
+  served        the metric's served half: the first 400 requests of the
+                serving stream (Poisson arrivals, 70% PRP, n = 2^10..2^22,
+                2000 iterations each) through m2_server in real time, p50/p99
+                completion latency and tests/s against a measured FIFO run
+                of the same requests one at a time; every Res64 the same
+                alone and in the mix;
+  cpu_baseline  the exact CPU restatement of the reference's algorithm
+                (oracle/) on all host threads, with the one-thread figure
+                beside it (the reference's own OpenCL path needs a CPU OpenCL
+                device, which the GPU host does not have).
 """
 from __future__ import annotations
 
@@ -73,9 +85,28 @@ def load_valu(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(p: int, mode: str, fixture_case, iters: int):
-    """Times the oracle (CPU restatement, 1 thread) for `iters` iterations and
-    checks its Res64 against the fixture when the fixture has that checkpoint."""
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"nproc": affinity, "cpu_model": model, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(p: int, mode: str, iters_mt: int, iters_1: int, gpu_res64):
+    """Times the oracle (the exact CPU restatement of the reference's
+    algorithm) on all host threads for `iters_mt` iterations, then on one
+    thread for `iters_1` more, and checks its Res64 against the GPU's at the
+    same iteration (gpu_res64(k))."""
     so = os.path.join(ROOT, "oracle", "liboracle.so")
     if not os.path.exists(so):
         import subprocess
@@ -83,23 +114,196 @@ def cpu_baseline(p: int, mode: str, fixture_case, iters: int):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_ctypes import Oracle
     o = Oracle(so)
+    threads = o.L.m2o_set_threads(0)
     t = o.L.m2o_test_new(p, 1 if mode == "prp" else 0, 64)
     try:
-        o.L.m2o_test_run(t, 1)  # first iteration outside the sample (cache warm-up)
+        o.L.m2o_test_run(t, 1)  # first iteration outside the sample (page-in, cache warm-up)
         t0 = time.perf_counter()
-        o.L.m2o_test_run(t, iters)
-        dt = time.perf_counter() - t0
+        o.L.m2o_test_run(t, iters_mt)
+        dt_mt = time.perf_counter() - t0
+        o.L.m2o_set_threads(1)
+        t0 = time.perf_counter()
+        o.L.m2o_test_run(t, iters_1)
+        dt_1 = time.perf_counter() - t0
+        o.L.m2o_set_threads(threads)
+        k = 1 + iters_mt + iters_1
         res = "%016X" % o.L.m2o_test_res64(t)
     finally:
         o.L.m2o_test_free(t)
-    ok = None
-    if fixture_case and str(iters + 1) in fixture_case["checkpoints"]:
-        ok = fixture_case["checkpoints"][str(iters + 1)]["res64"] == res
-    return {"value": dt / iters * 1e6, "unit": "µs/iter", "cores": 1, "kind": "port",
-            "sample": "oracle/m2_oracle.c (CPU restatement of the reference's radix-4 IBDWT, 1 thread), "
-                      "iterations 2..%d of the same %s test on this host; Res64 at %d %s" % (
-                          iters + 1, mode.upper(), iters + 1,
-                          {True: "matches", False: "MISMATCH", None: "unchecked"}[ok])}
+    want = gpu_res64(k)
+    host = host_info()
+    return {"value": round(dt_mt / iters_mt * 1e6, 1), "unit": "µs/iter", "cores": threads, "kind": "port",
+            "sample": "oracle/m2_oracle.c (exact CPU restatement of the reference's radix-4 IBDWT, OpenMP over "
+                      "each stage's independent butterflies), iterations 2..%d of the same %s test on %d host "
+                      "threads (%.1f s), then %d..%d on 1 thread; Res64 at %d %s the GPU's" % (
+                          1 + iters_mt, mode.upper(), threads, dt_mt, 2 + iters_mt, k, k,
+                          "matches" if res == want else "DIFFERS FROM"),
+            "single_thread": {"value": round(dt_1 / iters_1 * 1e6, 1), "unit": "µs/iter", "cores": 1},
+            "host": host,
+            "reason": "the reference's OpenCL path needs a CPU OpenCL device (PoCL), which the GPU host does not "
+                      "have, and it enumerates GPU devices only (mersenne2_gpu.cpp:3743-3752); this is the exact "
+                      "restatement of the same algorithm on the same host cores",
+            "res64_match": res == want}
+
+
+def batched_leg(ctx, seed: int, cap: int, sample: int):
+    """512 PRP tests (exponents log-uniform over n = 2^8..2^16) for
+    min(p, cap) iterations each: all in one m2_batch, against the plain form
+    (one m2_test after another, each created, run and read alone) timed on a
+    seeded sample of the same tests. Wall times include creating the tests
+    and reading every Res64 in both forms."""
+    import random
+    import mersenne2gpu_amd as m2
+    from mersenne2gpu_amd import workloads as W
+    ps = W.batch_exponents(seed, 512)
+    stops = [min(p, cap) for p in ps]
+    t0 = time.perf_counter()
+    b = m2.Batch(ctx, ps, "prp", iters=stops)
+    b.run(0)
+    b.sync()
+    got = [b.res64(i) for i in range(len(ps))]
+    t_batched = time.perf_counter() - t0
+    b.close()
+    idx = sorted(random.Random(seed).sample(range(len(ps)), sample))
+    t0 = time.perf_counter()
+    plain_diff = 0
+    for i in idx:
+        with m2.Test(ctx, ps[i], "prp") as t:
+            t.run(stops[i])
+            plain_diff += t.res64() != got[i]
+    t_plain = time.perf_counter() - t0
+    checked = mismatched = 0
+    fx = _fixture("c4.json")
+    if fx and fx.get("seed") == seed:
+        byp = {c["p"]: c for c in fx["cases"]}
+        for i, p in enumerate(ps):
+            c = byp.get(p)
+            if c and str(stops[i]) in c["checkpoints"]:
+                checked += 1
+                mismatched += got[i] != c["checkpoints"][str(stops[i])]["res64"]
+    lns = [W.transformsize(p) for p in ps]
+    batched_tps = len(ps) / t_batched
+    plain_tps = len(idx) / t_plain
+    log("batched: %d tests in %.3f s (%.1f tests/s); plain sample %d in %.3f s (%.2f tests/s)" % (
+        len(ps), t_batched, batched_tps, len(idx), t_plain, plain_tps))
+    return {"tests_per_s": round(batched_tps, 2), "plain_tests_per_s": round(plain_tps, 3),
+            "vs_plain": round(batched_tps / plain_tps, 2), "batched_s": round(t_batched, 3),
+            "plain_sample_s": round(t_plain, 3), "plain_sample": len(idx),
+            "res64_checked_fixture": checked, "res64_mismatched_fixture": mismatched,
+            "plain_res64_mismatched": plain_diff,
+            "workload": "512 PRP tests, seed %d, log-uniform over n=2^8..2^16 (%s), min(p, %d) iterations each; "
+                        "plain = one m2_test per test, timed on a seeded sample of %d" % (
+                            seed, ",".join("2^%d:%d" % (ln, lns.count(ln)) for ln in sorted(set(lns))), cap,
+                            len(idx))}
+
+
+def _pct(xs, q):
+    xs = sorted(xs)
+    k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[k]
+
+
+def served_leg(ctx, seed: int, count: int, iters: int, budget_gb: float, slice_iters: int):
+    """The first `count` requests of the serving stream, submitted at their
+    arrival times to m2_server (memory budget, time slices), against the
+    plain form measured the same way: the same requests served one at a
+    time in arrival order (FIFO), each an m2_test created, run and read
+    alone. Completion latency = completion - arrival."""
+    import mersenne2gpu_amd as m2
+    from mersenne2gpu_amd import workloads as W
+    reqs = W.serve_stream(seed, 2000, 40.0, iters)[:count]
+    srv = m2.Server(ctx, hbm_budget_bytes=int(budget_gb * 1e9), slice_iters=slice_iters)
+    ids, done = {}, {}
+    nxt = 0
+    peak_active = 0
+    t0 = time.perf_counter()
+    while len(done) < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and reqs[nxt][0] <= now:
+            _, p, mode, k = reqs[nxt]
+            ids[srv.submit(p, mode, k)] = nxt
+            nxt += 1
+        busy = srv.step()
+        tnow = time.perf_counter() - t0
+        for rid, r64, _ in srv.poll():
+            done[ids[rid]] = (tnow, r64)
+        peak_active = max(peak_active, srv.stats()["active"])
+        if not busy and nxt < len(reqs):
+            wait = reqs[nxt][0] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(min(wait, 0.005))
+    makespan = time.perf_counter() - t0
+    srv.close()
+    lat = [done[i][0] - reqs[i][0] for i in range(len(reqs))]
+    # capacity: the same requests all submitted at once
+    srv = m2.Server(ctx, hbm_budget_bytes=int(budget_gb * 1e9), slice_iters=slice_iters)
+    t0 = time.perf_counter()
+    cap_ids = {srv.submit(p, mode, k): i for i, (_, p, mode, k) in enumerate(reqs)}
+    cap_r64 = {}
+    while True:
+        busy = srv.step()
+        for rid, r64, _ in srv.poll():
+            cap_r64[cap_ids[rid]] = r64
+        if not busy and len(cap_r64) == len(reqs):
+            break
+        if time.perf_counter() - t0 > 600:
+            raise RuntimeError("served capacity run did not finish")
+    cap_makespan = time.perf_counter() - t0
+    srv.close()
+    # plain: FIFO, one m2_test at a time, in real time; each test's own
+    # service time (create, run, read) is kept for the capacity comparison
+    plain_lat, plain_r64, plain_service = [], [], []
+    t0 = time.perf_counter()
+    for arr, p, mode, k in reqs:
+        wait = arr - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        s0 = time.perf_counter()
+        with m2.Test(ctx, p, mode) as t:
+            t.run(k)
+            plain_r64.append(t.res64())
+        plain_service.append(time.perf_counter() - s0)
+        plain_lat.append(time.perf_counter() - t0 - arr)
+    plain_makespan = time.perf_counter() - t0
+    differ = sum(plain_r64[i] != done[i][1] or plain_r64[i] != cap_r64.get(i) for i in range(len(reqs)))
+    checked = mismatched = 0
+    fx = _fixture("c5.json")
+    if fx and fx.get("seed") == seed:
+        want = {(c["p"], c["mode"], int(k)): v["res64"] for c in fx["cases"] for k, v in c["checkpoints"].items()}
+        for i, (_, p, mode, k) in enumerate(reqs):
+            if (p, mode, k) in want:
+                checked += 1
+                mismatched += done[i][1] != want[(p, mode, k)]
+    lns = [W.transformsize(p) for _, p, _, _ in reqs]
+    out = {"tests_per_s": round(len(reqs) / makespan, 2), "p50_s": round(_pct(lat, 50), 4),
+           "p99_s": round(_pct(lat, 99), 4), "makespan_s": round(makespan, 3), "peak_active": peak_active,
+           "plain": {"tests_per_s": round(len(reqs) / plain_makespan, 2), "p50_s": round(_pct(plain_lat, 50), 4),
+                     "p99_s": round(_pct(plain_lat, 99), 4), "makespan_s": round(plain_makespan, 3),
+                     "how": "measured: FIFO in arrival order, one m2_test at a time"},
+           "tests_per_s_vs_plain": round(plain_makespan / makespan, 2),
+           "p99_vs_plain": round(_pct(plain_lat, 99) / _pct(lat, 99), 2),
+           "capacity": {"tests_per_s": round(len(reqs) / cap_makespan, 2),
+                        "plain_tests_per_s": round(len(reqs) / sum(plain_service), 2),
+                        "vs_plain": round(sum(plain_service) / cap_makespan, 2),
+                        "how": "all requests submitted at once; plain = the same tests back to back (sum of the "
+                               "FIFO run's measured service times)"},
+           "res64_same_alone_and_in_mix": differ == 0, "res64_differ": differ,
+           "res64_checked_fixture": checked, "res64_mismatched_fixture": mismatched,
+           "workload": "first %d requests of the seed-%d serving stream (Poisson 40/s over %.1f s, 70%% PRP; %s), "
+                       "%d iterations each, %.0f GB budget, %d-iteration slices" % (
+                           len(reqs), seed, reqs[-1][0],
+                           ",".join("2^%d:%d" % (ln, lns.count(ln)) for ln in sorted(set(lns))), iters, budget_gb,
+                           slice_iters)}
+    log("served: %s" % json.dumps(out))
+    return out
+
+
+def _fixture(name):
+    path = os.path.join(ROOT, "tests", "golden", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 # Workloads: exponent range, mode, exact fixture and its checked iterations.
@@ -107,9 +311,9 @@ def cpu_baseline(p: int, mode: str, fixture_case, iters: int):
 # LL test is measured the same way on request.
 WORKLOADS = {
     "prp-n2p20": {"range": "C2_RANGE", "mode": "prp", "fixture": "c2.json", "checks": (500, 1000, 2000),
-                  "steps": 100000, "cpu_iters": 9, "traffic_key": "c2"},
+                  "steps": 100000, "cpu_iters": (100, 9), "traffic_key": "c2"},
     "ll-n2p24": {"range": "C3_RANGE", "mode": "ll", "fixture": "c3.json", "checks": (100, 1000),
-                 "steps": 5000, "cpu_iters": 2, "traffic_key": "c3"},
+                 "steps": 5000, "cpu_iters": (8, 1), "traffic_key": "c3"},
 }
 
 
@@ -121,16 +325,22 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--kernel-iters", type=int, default=200, help="iterations timed kernel by kernel")
-    ap.add_argument("--cpu-iters", type=int, default=None, help="oracle iterations in the CPU baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=None,
+                    help="oracle iterations on all host threads in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-served", action="store_true", help="skip the served leg")
+    ap.add_argument("--batch-cap", type=int, default=2000, help="iterations per batched test")
+    ap.add_argument("--batch-plain-sample", type=int, default=64, help="tests in the measured plain sample")
+    ap.add_argument("--serve-count", type=int, default=400, help="requests of the serving stream")
+    ap.add_argument("--serve-iters", type=int, default=2000, help="iterations per served request")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device-to-device 3 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     wl = WORKLOADS[args.workload]
     if args.steps is None:
         args.steps = wl["steps"]
-    if args.cpu_iters is None:
-        args.cpu_iters = wl["cpu_iters"]
+    cpu_iters = (args.cpu_iters, wl["cpu_iters"][1]) if args.cpu_iters else wl["cpu_iters"]
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -204,12 +414,24 @@ def main() -> int:
     # centre's 6 h), against the measured time
     modmuls = 2 * 49.5 * h
     t.close()
-    ctx.close()
 
+    def gpu_res64(k):
+        with m2.Test(ctx, p, mode) as tk:
+            tk.run(k)
+            return tk.res64()
+
+    batched = None if args.no_batched else batched_leg(ctx, args.seed, args.batch_cap, args.batch_plain_sample)
+    served = None if args.no_served else served_leg(ctx, args.seed, args.serve_count, args.serve_iters, 200.0, 1000)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(p, mode, fixture, args.cpu_iters)
+        cpu = cpu_baseline(p, mode, cpu_iters[0], cpu_iters[1], gpu_res64)
         log("cpu baseline:", cpu)
+    ctx.close()
+    traffic_src = None
+    if traffic is not None:
+        tj = os.path.join(ROOT, "profiles", "traffic.json")
+        with open(tj) as f:
+            traffic_src = json.load(f).get("_source")
 
     out = {
         "metric": METRIC,
@@ -236,16 +458,26 @@ def main() -> int:
         "per_kernel_us": {k: round(v, 3) for k, v in per_kernel_us.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "bytes_per_launch": bytes_dom, "us_per_launch": round(per_kernel_us[dom], 3)},
         "int_roofline": {"modmuls_per_iter": int(modmuls),
                          "achieved_gmodmul_s": round(modmuls / (us_per_iter * 1e-6) / 1e9, 1),
                          "kernel": dom, "valu_busy": load_valu(dom, wl["traffic_key"]),
                          "valu_busy_source": "profiles/valu.json: rocprofv3 SQ counters, VALU instructions "
                                              "per SIMD per 4-cycle issue slot"},
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    return 0 if parity_ok else 1
+    ok = parity_ok
+    if batched:
+        ok = ok and batched["res64_mismatched_fixture"] == 0 and batched["plain_res64_mismatched"] == 0
+    if served:
+        ok = ok and served["res64_differ"] == 0 and served["res64_mismatched_fixture"] == 0
+    if cpu:
+        ok = ok and cpu["res64_match"]
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

@@ -10,11 +10,34 @@
  * Deliberate difference: digit widths of 0 bits (only for p < 8, where
  * q = p >> ln is 0) are handled exactly; the reference shifts a 64-bit word
  * by 64 in that case (digit_adc, :1982-1993), which is undefined.
+ *
+ * Built with OpenMP, the loops whose iterations are independent (butterflies
+ * of one stage, centre pairs, weights, the carry blocks' prepare step) are
+ * shared among m2o_set_threads() host threads, as the reference's work-items
+ * are among a CPU OpenCL device's cores; the results do not depend on the
+ * thread count. The carry drain and the LL subtract stay serial, as in the
+ * reference (single work-item kernels, :2967, :3006).
  */
 #include "m2_oracle.h"
 
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Work below this many elements per loop stays on one thread. */
+#define PAR_MIN 4096
+
+int m2o_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 typedef unsigned __int128 u128;
 
@@ -217,10 +240,11 @@ void m2o_twiddles(int ln, m2o_gf *w) {
 /* One radix-4 stage with s blocks of 4m (forward4, :732-747): inputs 1,2,3 are
  * multiplied by w[2(s+j)], w[s+j], w[h+s+j], then the (+-1, +-i) butterfly. */
 static void fwd4(m2o_gf *z, const m2o_gf *w, uint64_t h, uint64_t s, uint64_t m) {
+#pragma omp parallel for collapse(2) if (s * m >= PAR_MIN)
     for (uint64_t j = 0; j < s; ++j) {
-        const m2o_gf w1 = w[2 * (s + j)], w2 = w[s + j], w3 = w[h + s + j];
-        const uint64_t b = j * 4 * m;
         for (uint64_t i = 0; i < m; ++i) {
+            const m2o_gf w1 = w[2 * (s + j)], w2 = w[s + j], w3 = w[h + s + j];
+            const uint64_t b = j * 4 * m;
             m2o_gf u0 = z[b + i];
             m2o_gf u1 = gf_mul(z[b + m + i], w1);
             m2o_gf u2 = gf_mul(z[b + 2 * m + i], w2);
@@ -236,10 +260,11 @@ static void fwd4(m2o_gf *z, const m2o_gf *w, uint64_t h, uint64_t s, uint64_t m)
 
 /* Inverse radix-4 stage (backward4, :951-962). */
 static void bwd4(m2o_gf *z, const m2o_gf *w, uint64_t h, uint64_t s, uint64_t m) {
+#pragma omp parallel for collapse(2) if (s * m >= PAR_MIN)
     for (uint64_t j = 0; j < s; ++j) {
-        const m2o_gf w1 = w[2 * (s + j)], w2 = w[s + j], w3 = w[h + s + j];
-        const uint64_t b = j * 4 * m;
         for (uint64_t i = 0; i < m; ++i) {
+            const m2o_gf w1 = w[2 * (s + j)], w2 = w[s + j], w3 = w[h + s + j];
+            const uint64_t b = j * 4 * m;
             m2o_gf u0 = z[b + i], u1 = z[b + m + i], u2 = z[b + 2 * m + i], u3 = z[b + 3 * m + i];
             m2o_gf v0 = gf_add(u0, u1), v1 = gf_sub(u0, u1), v2 = gf_add(u2, u3), v3 = gf_sub(u3, u2);
             z[b + i] = gf_add(v0, v2);
@@ -259,6 +284,7 @@ static void forward_plan(int ln, m2o_gf *z, const m2o_gf *w) {
     uint64_t current_m = n / 4, s = 1;
     for (; current_m > 1; current_m /= 4, s *= 4) fwd4(z, w, h, s, current_m / 2);
     if (current_m == 1) {
+#pragma omp parallel for if (h / 2 >= PAR_MIN)
         for (uint64_t j = 0; j < h / 2; ++j) {
             m2o_gf u0 = z[2 * j], u1 = gf_mul(z[2 * j + 1], w[h / 2 + j]);
             z[2 * j] = gf_add(u0, u1);
@@ -275,6 +301,7 @@ static void inverse_plan(int ln, m2o_gf *z, const m2o_gf *w) {
     for (; current_m > 1; current_m /= 4, s *= 4) {
     }
     if (current_m == 1) {
+#pragma omp parallel for if (h / 2 >= PAR_MIN)
         for (uint64_t j = 0; j < h / 2; ++j) {
             m2o_gf u0 = z[2 * j], u1 = z[2 * j + 1];
             z[2 * j] = gf_add(u0, u1);
@@ -290,6 +317,8 @@ static void inverse_plan(int ln, m2o_gf *z, const m2o_gf *w) {
  * twiddle w[h/2+j] is the h-th root of unity of position k. */
 void m2o_square_half(int ln, m2o_gf *z, const m2o_gf *w) {
     const uint64_t h = ((uint64_t)1) << (ln - 1);
+    /* pairs (k, mk) are disjoint: k even, mk odd and >= 3 for k >= 2 */
+#pragma omp parallel for if (h / 2 >= PAR_MIN)
     for (uint64_t j = 0; j < h / 2; ++j) {
         uint64_t k = 2 * j, mk = 0;
         if (k != 0) {
@@ -315,6 +344,7 @@ void m2o_inverse(int ln, m2o_gf *z, const m2o_gf *w) { inverse_plan(ln, z, w); }
 /* z_k <- (d_2k * 2^w[2k], d_2k+1 * 2^w[2k+1]) in each field (weight, :517). */
 void m2o_weight(int ln, m2o_gf *z, const uint8_t *w61, const uint8_t *w31) {
     const uint64_t h = ((uint64_t)1) << (ln - 1);
+#pragma omp parallel for if (h >= PAR_MIN)
     for (uint64_t k = 0; k < h; ++k) {
         z[k].s0 = m2o_shl61(z[k].s0, w61[2 * k]);
         z[k].s1 = m2o_shl61(z[k].s1, w61[2 * k + 1]);
@@ -328,6 +358,7 @@ void m2o_weight(int ln, m2o_gf *z, const uint8_t *w61, const uint8_t *w31) {
 void m2o_unweight(int ln, m2o_gf *z, const uint8_t *w61, const uint8_t *w31) {
     const uint64_t h = ((uint64_t)1) << (ln - 1);
     const unsigned l1 = (unsigned)ln + 1u;
+#pragma omp parallel for if (h >= PAR_MIN)
     for (uint64_t k = 0; k < h; ++k) {
         z[k].s0 = shr61(z[k].s0, w61[2 * k] + l1);
         z[k].s1 = shr61(z[k].s1, w61[2 * k + 1] + l1);
@@ -374,6 +405,7 @@ struct m2o_test {
 static void block_carry(m2o_test *t) {
     const uint32_t nb = t->nblocks, bp = t->carry_pairs;
     /* prepare: Garner + zero-carry-in normalisation of each block */
+#pragma omp parallel for if (t->h >= PAR_MIN)
     for (uint32_t b = 0; b < nb; ++b) {
         u128 c = 0;
         uint64_t k0 = (uint64_t)b * bp, k1 = k0 + bp < t->h ? k0 + bp : t->h;

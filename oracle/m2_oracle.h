@@ -54,6 +54,10 @@ void m2o_unweight(int ln, m2o_gf *z, const uint8_t *w61, const uint8_t *w31); /*
 /* Garner CRT of (x mod M61, x mod M31) into x < 2^92 (:2018-2030); out[0]=lo, out[1]=hi. */
 void m2o_garner(uint64_t s, uint32_t t, uint64_t out[2]);
 
+/* Host threads the independent loops are shared among (n > 0 sets it; 0
+ * leaves it); returns the count in use (1 without OpenMP). */
+int m2o_set_threads(int n);
+
 /* ---- whole test ---- */
 typedef struct m2o_test m2o_test;
 /* mode: 0 = LL (start 4), 1 = PRP base 3 (start 3), :3908. carry_pairs: block size (:3946-3967). */

@@ -6,6 +6,7 @@ Python integers), never from the transform code under test. Usage:
     python tests/golden/gen_fixtures.py c2                   # a few minutes (GMP)
     python tests/golden/gen_fixtures.py c3                   # ~20-30 minutes (GMP)
     python tests/golden/gen_fixtures.py c5                   # a few minutes (GMP)
+    python tests/golden/gen_fixtures.py c5big                # ~10 minutes on 8 cores (GMP)
     python tests/golden/gen_fixtures.py mid                  # a few minutes (GMP)
 """
 from __future__ import annotations
@@ -147,6 +148,39 @@ def gen_c5():
     write("c5.json", {"desc": "serving stream seed 1: exact Res64 of requests with n <= 2^14 among the first "
                               "400, and of the first six with n = 2^16, at their iteration budget",
                       "seed": 1, "cases": cases})
+
+
+# Requests with n >= 2^17 in the serving test stream run this many iterations
+# (the rest keep the stream's 2000), so that exact values stay cheap to make.
+C5_BIG_LN, C5_BIG_ITERS = 17, 100
+
+
+def _case_args(a):
+    return case(*a)
+
+
+def gen_c5big():
+    # Adds to c5.json the exact Res64 of every request with n >= 2^17 among the
+    # first 400 of the serving stream (seed 1), at C5_BIG_ITERS iterations.
+    import multiprocessing
+    reqs = W.serve_stream(1)
+    todo, seen = [], set()
+    for i, (_, p, mode, _) in enumerate(reqs[:400]):
+        if W.transformsize(p) >= C5_BIG_LN and (p, mode) not in seen:
+            seen.add((p, mode))
+            todo.append((p, mode, [C5_BIG_ITERS]))
+    with multiprocessing.Pool(min(8, os.cpu_count() or 1)) as pool:
+        big = pool.map(_case_args, todo, chunksize=1)
+    path = os.path.join(HERE, "c5.json")
+    with open(path) as f:
+        fx = json.load(f)
+    have = {(c["p"], c["mode"]) for c in fx["cases"]}
+    fx["cases"] += [c for c in big if (c["p"], c["mode"]) not in have]
+    fx["big_ln"], fx["big_iters"] = C5_BIG_LN, C5_BIG_ITERS
+    fx["desc"] = ("serving stream seed 1: exact Res64 of requests with n <= 2^14 among the first 400, and of the "
+                  "first six with n = 2^16, at their iteration budget; of every request with n >= 2^17 among "
+                  "the first 400 at %d iterations" % C5_BIG_ITERS)
+    write("c5.json", fx)
 
 
 if __name__ == "__main__":

@@ -29,6 +29,7 @@ class Oracle:
             ("m2o_test_verdict", i, [ctypes.c_void_p]), ("m2o_test_ln", i, [ctypes.c_void_p]),
             ("m2o_test_digits", None, [ctypes.c_void_p, ctypes.POINTER(u64)]),
             ("m2o_digits_residue", i, [u32, i, ctypes.POINTER(u64), ctypes.c_char_p, ctypes.c_size_t]),
+            ("m2o_set_threads", i, [i]),
         ]:
             f = getattr(L, name)
             f.restype = res

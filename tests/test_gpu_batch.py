@@ -274,3 +274,74 @@ def test_fused_pass_reports_a_wait_time_out(gpu_ctx):
     with m2.Test(gpu_ctx, c2["p"], "prp", graph_iters=1) as t:
         t.run(10)
         assert t.res64() == c2["checkpoints"]["10"]["res64"]
+
+
+def _c5_requests():
+    """The first 400 requests of the serving workload's stream (seed 1:
+    Poisson arrivals, 70% PRP, n = 2^10..2^22), with the stream's 2000
+    iterations per request except n >= 2^17, which run big_iters (100), and
+    the exact Res64 of each (tests/golden/c5.json)."""
+    from mersenne2gpu_amd import workloads as W
+    fx = load_fixture("c5.json")
+    if "big_iters" not in fx:
+        pytest.skip("c5.json lacks the n >= 2^17 entries (gen_fixtures.py c5big)")
+    want = {(c["p"], c["mode"], int(k)): v["res64"] for c in fx["cases"] for k, v in c["checkpoints"].items()}
+    reqs = []
+    for _, p, mode, k in W.serve_stream(fx["seed"])[:400]:
+        if W.transformsize(p) >= fx["big_ln"]:
+            k = fx["big_iters"]
+        reqs.append((p, mode, k, want.get((p, mode, k))))
+    return reqs
+
+
+@pytest.mark.parametrize("budget_gb", [200.0, 1.0])
+def test_server_c5_stream(gpu_ctx, budget_gb):
+    """The serving workload's stream through m2_server: 200 GB (the workload's
+    budget: everything admitted at once) and 1 GB (admission waits for
+    memory; the n = 2^22 lanes take a quarter of it each). Every completion is
+    checked against the exact value where the fixture holds it (all n <= 2^14
+    requests, six at n = 2^16 and every n >= 2^17 one, including the n = 2^20
+    and 2^22 lanes), and every one against the same request run alone."""
+    from mersenne2gpu_amd import workloads as W
+    reqs = _c5_requests()
+    lns = [W.transformsize(p) for p, _, _, _ in reqs]
+    assert 20 in lns and 22 in lns
+    budget = int(budget_gb * 1e9)
+    print("c5 stream: %d requests, %d with exact values, budget %.0f GB" % (
+        len(reqs), sum(w is not None for _, _, _, w in reqs), budget_gb), flush=True)
+    got = {}
+    with m2.Server(gpu_ctx, hbm_budget_bytes=budget, slice_iters=1000) as s:
+        ids = {s.submit(p, mode, k): i for i, (p, mode, k, _) in enumerate(reqs)}
+        steps = waited = 0
+        while True:
+            busy = s.step()
+            steps += 1
+            st = s.stats()
+            assert st["bytes_in_use"] <= budget or st["active"] == 1
+            waited += st["queued"] > 0 and st["active"] > 0
+            for rid, r64, _ in s.poll():
+                got[ids[rid]] = r64
+            if not busy:
+                break
+            assert steps < 200000
+        for rid, r64, _ in s.poll():
+            got[ids[rid]] = r64
+    assert len(got) == len(reqs)
+    if budget_gb < 10:
+        assert waited > 0
+    checked = 0
+    for i, (p, mode, k, want) in enumerate(reqs):
+        if want is not None:
+            checked += 1
+            assert got[i] == want, (i, p, mode, k)
+    assert checked >= 300
+    # alone: one request of each length, and every n >= 2^20 one
+    seen = set()
+    for i, (p, mode, k, _) in enumerate(reqs):
+        ln = lns[i]
+        if ln in seen and ln < 20:
+            continue
+        seen.add(ln)
+        with m2.Test(gpu_ctx, p, mode) as t:
+            t.run(k)
+            assert t.res64() == got[i], (i, p, mode, k)
