@@ -223,6 +223,12 @@ int upload_digits(TestObj* t, const int64_t* digits, hipStream_t st) {
     HIPCHK(hipMemcpyAsync(t->desc.G, zero.data(), zero.size() * sizeof(Carry128), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
   }
+  if (t->desc.sync) {
+    // a new state: clear the fused pass's time-out word (the per-strip launch
+    // counters keep counting)
+    const uint32_t zero = 0;
+    HIPCHK(hipMemcpyAsync(t->desc.sync + outer_strips(P), &zero, sizeof zero, hipMemcpyHostToDevice, st));
+  }
   HIPCHK(hipStreamSynchronize(st));
   return M2_OK;
 }
@@ -714,7 +720,9 @@ int m2_test_is_prime(m2_test* t, int* verdict) {
 int m2_test_set_residue(m2_test* t, const uint8_t* buf, size_t len, uint64_t iteration) {
   if (!t || !buf) return fail(M2_E_INVAL, "null pointer");
   HIPCHK(hipSetDevice(t->ctx->device));
-  int rc = test_sync(t);
+  // the state is replaced, so a fused-pass time-out in it does not matter
+  // (upload_digits clears the time-out word)
+  int rc = t->group.sync();
   if (rc) return rc;
   std::vector<int64_t> digits(2ull * t->obj->cls->P.h);
   digits_from_residue(t->obj->p, t->obj->tt.bitpos, t->obj->tt.dig, buf, len, digits.data());

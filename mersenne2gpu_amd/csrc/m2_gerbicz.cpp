@@ -85,7 +85,12 @@ int m2_test_run_checked(m2_test* t, uint64_t iters, uint32_t block, uint32_t blo
   if (rc) return rc;
   if (mode != M2_MODE_PRP) return test_error(M2_E_INVAL, "the Gerbicz check applies to PRP tests");
   Checked c((uint32_t)p);
-  if ((rc = m2_test_create(ctx, p, M2_MODE_PRP, nullptr, &c.s))) return rc;
+  // The scratch test keeps its outer passes in separate launches: its
+  // squarings overlap the test's own launches on another stream, and a fused
+  // outer pass needs all its workgroups resident at once.
+  m2_opts so{};
+  so.no_fuse = 1;
+  if ((rc = m2_test_create(ctx, p, M2_MODE_PRP, &so, &c.s))) return rc;
   uint64_t start = 0;
   if ((rc = m2_test_iteration(t, &start))) return rc;
   uint64_t nerr = 0;
@@ -116,8 +121,16 @@ int m2_test_run_checked(m2_test* t, uint64_t iters, uint32_t block, uint32_t blo
   for (uint64_t k = vk; k < full_end;) {
     Num x;
     if (!queued && (rc = m2_test_run(t, block))) return rc;
-    if ((rc = c.read(t, x))) return rc;  // waits for the block
+    rc = c.read(t, x);  // waits for the block
     queued = false;
+    if (rc == M2_E_STATE) {
+      // a fused outer pass timed out waiting for a neighbour: the block's
+      // state is invalid, so it is treated as a failed check
+      if ((rc = roll_back())) return rc;
+      k = vk;
+      continue;
+    }
+    if (rc) return rc;
     k += block;
     if (++blocks_run == fault) {  // planted error (test hook): one flipped bit
       x[0] ^= 1ull << 5;

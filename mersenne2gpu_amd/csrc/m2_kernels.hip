@@ -54,28 +54,62 @@ constexpr int centre_tw_slots() {
 
 // Signed digit -> canonical field residues.
 __device__ __forceinline__ uint64_t digit61(int64_t d) { return d >= 0 ? (uint64_t)d : P61 - (uint64_t)(-d); }
+// x mod P31 in [0, P31] for |x| < 2^62 (digits are below 2^46): one fold
+// (x mod 2^31 + x >> 31 < 2 P31) and an unsigned min.
 __device__ __forceinline__ uint32_t mod31_u64(uint64_t x) {
-  uint64_t r = (x & P31) + (x >> 31);
-  r = (r & P31) + (r >> 31);
-  return (uint32_t)(r >= P31 ? r - P31 : r);
+  const uint32_t r = ((uint32_t)x & P31) + (uint32_t)(x >> 31);
+  return umin32(r, r - P31);
 }
 __device__ __forceinline__ uint32_t digit31(int64_t d) {
   return d >= 0 ? mod31_u64((uint64_t)d) : neg31(mod31_u64((uint64_t)(-d)));
 }
 
-// Unweight (x 2^-(w + ln + 1), which also removes 1/h and the centre's 4) and
-// Garner CRT (crt61_31: x in [0, M) read unsigned, a narrow band below M as
-// x - M), plus `ones` (the digit's all-ones value 2^width - 1 in LL tests,
-// else 0; see seg_normalise).
-__device__ __forceinline__ i128 crt_digit(uint64_t s, uint32_t t, uint32_t w61, uint32_t w31, const DigitGen& D,
-                                          uint64_t ones) {
-  uint32_t a61 = w61 + D.k61;
-  a61 -= (a61 >= 61u) ? 61u : 0u;
-  uint32_t a31 = w31 + D.k31;
-  a31 -= (a31 >= 31u) ? 31u : 0u;
-  s = rot61(s, 61u - a61);  // a rotation by 61 (31) is the identity: no select for a = 0
-  t = rot31(t, 31u - a31);
-  return crt61_31(s, t, ones);
+// Digit metadata of one carry segment (the 8 digits from j0), walked once and
+// kept in registers for the segment's normalise, ripple and weight steps. Per
+// digit: its weight shifts w61 (bits 0-5) and w31 (6-10), the rotations that
+// unweight it, 61 - ((w61 + ln + 1) mod 61) (11-16) and 31 - ((w31 + ln + 1)
+// mod 31) (17-21), which also remove 1/h and the centre's factor 4 (a
+// rotation by 61 or 31 is the identity), and bit 22: its width is q + 1
+// (else q).
+struct SegMeta {
+  uint32_t m[2 * SEG_ELEMS];
+};
+__device__ __forceinline__ SegMeta seg_meta(const DigitGen& D, uint32_t j0) {
+  SegMeta M;
+  DWalk w = dwalk_at(D, j0);
+#pragma unroll
+  for (int i = 0; i < 2 * SEG_ELEMS; ++i) {
+    const uint32_t w61 = dwalk_w61(w), w31 = dwalk_w31(w);
+    uint32_t a61 = w61 + D.k61;
+    a61 -= (a61 >= 61u) ? 61u : 0u;
+    uint32_t a31 = w31 + D.k31;
+    a31 -= (a31 >= 31u) ? 31u : 0u;
+    const uint32_t wide = dwalk_width(D, w) - D.q;
+    M.m[i] = w61 | (w31 << 6) | ((61u - a61) << 11) | ((31u - a31) << 17) | (wide << 22);
+    if (i + 1 < 2 * SEG_ELEMS) dwalk_next(D, w);
+  }
+  return M;
+}
+__device__ __forceinline__ uint32_t meta_w61(uint32_t m) { return m & 63u; }
+__device__ __forceinline__ uint32_t meta_w31(uint32_t m) { return (m >> 6) & 31u; }
+__device__ __forceinline__ uint32_t meta_u61(uint32_t m) { return (m >> 11) & 63u; }
+__device__ __forceinline__ uint32_t meta_u31(uint32_t m) { return (m >> 17) & 31u; }
+__device__ __forceinline__ uint32_t meta_wide(uint32_t m) { return m >> 22; }
+
+// Digit masks 2^q - 1 and 2^(q+1) - 1 of a test (uniform per test).
+struct WidthMasks {
+  uint64_t q, q1;
+  __device__ __forceinline__ explicit WidthMasks(const DigitGen& D)
+      : q((1ull << D.q) - 1ull), q1((2ull << D.q) - 1ull) {}
+  __device__ __forceinline__ uint64_t of(uint32_t m) const { return meta_wide(m) ? q1 : q; }
+};
+
+// Unweight (the rotations in the digit's metadata) and Garner CRT (crt61_31:
+// x in [0, M) read unsigned, a narrow band below M as x - M), plus `ones`
+// (the digit's all-ones value 2^width - 1 in LL tests, else 0; see
+// seg_normalise).
+__device__ __forceinline__ i128 crt_digit(uint64_t s, uint32_t t, uint32_t meta, uint64_t ones) {
+  return crt61_31(rot61(s, meta_u61(meta)), rot31(t, meta_u31(meta)), ones);
 }
 
 // acc >> w (arithmetic) for 0 <= w < 64, as two 64-bit shifts of the words.
@@ -84,12 +118,6 @@ __device__ __forceinline__ i128 shr_w(i128 acc, uint32_t w) {
   const int64_t hi = (int64_t)(acc >> 64);
   const uint64_t clo = (lo >> w) | (((uint64_t)hi << 1) << (63u - w));
   return (i128)((unsigned __int128)(uint64_t)(hi >> w) << 64 | clo);
-}
-
-// Digit of width w (< 64) and the carry out of acc.
-__device__ __forceinline__ int64_t norm_digit(i128 acc, uint32_t w, i128& carry) {
-  carry = shr_w(acc, w);
-  return (int64_t)((uint64_t)acc & ((1ull << w) - 1ull));
 }
 
 // Per-test buffers reach the kernels through the test descriptors, i.e. as
@@ -145,53 +173,47 @@ struct SegDigits {
 // 0's is the term plus 2^width - 3 >= 0 (width >= 2), so no sum, digit or
 // carry is ever negative and the next squaring's terms stay in the range the
 // CRT reads (crt61_31). PRP tests pass ll_ones = 0 and carry-in 0.
-__device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, int idx0, uint32_t j0,
-                                              const DigitGen& D, i128 carry, uint64_t ll_ones, SegDigits& s) {
-  DWalk w = dwalk_at(D, j0);
+__device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, int idx0, const SegMeta& M,
+                                              const WidthMasks& wm, uint32_t q, i128 carry, uint64_t ll_ones,
+                                              SegDigits& s) {
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
     const e61 v = L61[PADX(idx0 + i)];
     const e31 y = L31[PADX(idx0 + i)];
-    const uint32_t wa = dwalk_width(D, w);
-    i128 xa = crt_digit(v.a, y.a, dwalk_w61(w), dwalk_w31(w), D, ((1ull << wa) - 1ull) & ll_ones);
-    dwalk_next(D, w);
-    const uint32_t wb = dwalk_width(D, w);
-    i128 xb = crt_digit(v.b, y.b, dwalk_w61(w), dwalk_w31(w), D, ((1ull << wb) - 1ull) & ll_ones);
-    dwalk_next(D, w);
-    s.d[2 * i] = norm_digit(xa + carry, wa, carry);
-    s.d[2 * i + 1] = norm_digit(xb + carry, wb, carry);
+    const uint32_t ma = M.m[2 * i], mb = M.m[2 * i + 1];
+    const uint64_t ka = wm.of(ma), kb = wm.of(mb);
+    const i128 xa = crt_digit(v.a, y.a, ma, ka & ll_ones) + carry;
+    s.d[2 * i] = (int64_t)((uint64_t)xa & ka);
+    carry = shr_w(xa, q + meta_wide(ma));
+    const i128 xb = crt_digit(v.b, y.b, mb, kb & ll_ones) + carry;
+    s.d[2 * i + 1] = (int64_t)((uint64_t)xb & kb);
+    carry = shr_w(xb, q + meta_wide(mb));
   }
   return carry;
 }
 
 // Add carry c at the segment's first digit; the top digit absorbs the rest.
-__device__ __forceinline__ void seg_ripple(SegDigits& s, uint32_t j0, const DigitGen& D, i128 c) {
-  if (c == 0) return;
-  DWalk w = dwalk_at(D, j0);
+// A wave skips the remaining digits once no lane has a carry left.
+__device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const WidthMasks& wm, uint32_t q, i128 c) {
 #pragma unroll
   for (int i = 0; i < 2 * SEG_ELEMS - 1; ++i) {
     if (c != 0) {
-      i128 acc = (i128)s.d[i] + c;
-      s.d[i] = norm_digit(acc, dwalk_width(D, w), c);
+      const i128 acc = (i128)s.d[i] + c;
+      s.d[i] = (int64_t)((uint64_t)acc & wm.of(M.m[i]));
+      c = shr_w(acc, q + meta_wide(M.m[i]));
     }
-    dwalk_next(D, w);
   }
   s.d[2 * SEG_ELEMS - 1] += (int64_t)c;
 }
 
 // The segment's digits -> weighted field elements at LDS index idx0.
-__device__ __forceinline__ void seg_weight(const SegDigits& s, e61* L61, e31* L31, int idx0, uint32_t j0,
-                                           const DigitGen& D) {
-  DWalk w = dwalk_at(D, j0);
+__device__ __forceinline__ void seg_weight(const SegDigits& s, e61* L61, e31* L31, int idx0, const SegMeta& M) {
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
     const int64_t da = s.d[2 * i], db = s.d[2 * i + 1];
-    const uint32_t a61 = dwalk_w61(w), a31 = dwalk_w31(w);
-    dwalk_next(D, w);
-    const uint32_t b61 = dwalk_w61(w), b31 = dwalk_w31(w);
-    dwalk_next(D, w);
-    L61[PADX(idx0 + i)] = {rot61(digit61(da), a61), rot61(digit61(db), b61)};
-    L31[PADX(idx0 + i)] = {rot31(digit31(da), a31), rot31(digit31(db), b31)};
+    const uint32_t ma = M.m[2 * i], mb = M.m[2 * i + 1];
+    L61[PADX(idx0 + i)] = {rot61(digit61(da), meta_w61(ma)), rot61(digit61(db), meta_w61(mb))};
+    L31[PADX(idx0 + i)] = {rot31(digit31(da), meta_w31(ma)), rot31(digit31(db), meta_w31(mb))};
   }
 }
 
@@ -908,8 +930,10 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
   const uint32_t j = 2u * (uint32_t)idx0;
   SegDigits sd;
   if (mine) seg_load(z61 + idx0, sd);
+  const SegMeta M = seg_meta(D, j);
+  const WidthMasks wm(D);
   for (uint32_t it = 0; it < td.iters; ++it) {
-    if (mine) seg_weight(sd, L61, L31, idx0, j, D);
+    if (mine) seg_weight(sd, L61, L31, idx0, M);
     __syncthreads();
     lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31);
     square_self<true, WAVE>(L61, L31, logH, 1, [&](int, int q) { return S61[q]; },
@@ -918,10 +942,10 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
     if (WAVE) __syncthreads();  // the carry reads both fields
     if (mine) {
       const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
-      store_carry(gs, s, seg_normalise(L61, L31, idx0, j, D, c0, td.mode == 0 ? ~0ull : 0ull, sd));
+      store_carry(gs, s, seg_normalise(L61, L31, idx0, M, wm, D.q, c0, td.mode == 0 ? ~0ull : 0ull, sd));
     }
     __syncthreads();
-    if (mine) seg_ripple(sd, j, D, load_carry(gs, (s + nseg - 1) % nseg));
+    if (mine) seg_ripple(sd, M, wm, D.q, load_carry(gs, (s + nseg - 1) % nseg));
   }
   if (mine) seg_store(z61 + idx0, sd);
 }
@@ -994,8 +1018,9 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
       const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
       SegDigits sd;
       seg_load(zd + (size_t)cg * (R * W) + idx0, sd);
-      if (v == 0) seg_ripple(sd, 2u * e0, D, carry_at(G, prev_strip_carry(cg, (uint32_t)r, R, NS)));
-      seg_weight(sd, L61, L31, idx0, 2u * e0, D);
+      const SegMeta M = seg_meta(D, 2u * e0);
+      if (v == 0) seg_ripple(sd, M, WidthMasks(D), D.q, carry_at(G, prev_strip_carry(cg, (uint32_t)r, R, NS)));
+      seg_weight(sd, L61, L31, idx0, M);
     }
   }
   ft.store(T61, T31, ntw);
@@ -1090,13 +1115,15 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   const int idx0 = r * W + v * SEG_ELEMS;
   const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
   SegDigits sd;
+  const SegMeta M = seg_meta(D, 2u * e0);
+  const WidthMasks wm(D);
   if (mine) {
     const i128 c0 = (mode == 0 && e0 == 0) ? (i128)(-2) : (i128)0;
-    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0, mode == 0 ? ~0ull : 0ull, sd));
+    store_carry(gs, it, seg_normalise(L61, L31, idx0, M, wm, D.q, c0, mode == 0 ? ~0ull : 0ull, sd));
   }
   __syncthreads();
   if (mine) {
-    if (v > 0) seg_ripple(sd, 2u * e0, D, load_carry(gs, it - 1));
+    if (v > 0) seg_ripple(sd, M, wm, D.q, load_carry(gs, it - 1));
     if (v == subs - 1) gst(G + cg * (uint32_t)R + (uint32_t)r, gs[it]);
 #pragma unroll
     for (int i = 0; i < SEG_ELEMS; ++i) L61[PADX(idx0 + i)] = e61{(uint64_t)sd.d[2 * i], (uint64_t)sd.d[2 * i + 1]};
@@ -1205,9 +1232,11 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   const int idx0 = r * W + v * SEG_ELEMS;
   const uint32_t e0 = ((uint32_t)r << logS) + col0 + (uint32_t)(v * SEG_ELEMS);
   SegDigits sd;
+  const SegMeta M = seg_meta(D, 2u * e0);
+  const WidthMasks wm(D);
   if (mine) {
     const i128 c0 = (mode == 0 && e0 == 0) ? (i128)(-2) : (i128)0;
-    store_carry(gs, it, seg_normalise(L61, L31, idx0, 2u * e0, D, c0, mode == 0 ? ~0ull : 0ull, sd));
+    store_carry(gs, it, seg_normalise(L61, L31, idx0, M, wm, D.q, c0, mode == 0 ? ~0ull : 0ull, sd));
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1220,11 +1249,12 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   // last carry is published; the row's first segment then takes the carry
   // the neighbour strip published; weight and the forward outer pass follow
   if (mine) {
-    if (v > 0) seg_ripple(sd, 2u * e0, D, load_carry(gs, it - 1));
+    if (v > 0) seg_ripple(sd, M, wm, D.q, load_carry(gs, it - 1));
     if (v == subs - 1) carry_put(cf, cg * (uint32_t)R + (uint32_t)r, load_carry(gs, it), epoch & 0xFFu);
     if (v == 0)
-      seg_ripple(sd, 2u * e0, D, carry_take(cf, prev_strip_carry(cg, r, R, NS), epoch & 0xFFu, spin_limit, sync + NS));
-    seg_weight(sd, L61, L31, idx0, 2u * e0, D);
+      seg_ripple(sd, M, wm, D.q,
+                 carry_take(cf, prev_strip_carry(cg, r, R, NS), epoch & 0xFFu, spin_limit, sync + NS));
+    seg_weight(sd, L61, L31, idx0, M);
   }
   __syncthreads();
   if (CA) {

@@ -256,22 +256,30 @@ def test_fused_and_separate_outer_passes_agree(gpu_ctx):
 def test_fused_pass_reports_a_wait_time_out(gpu_ctx):
     """The fused outer pass's bounded wait: with the poll limit forced to 0
     from the host, every fused launch reports a time-out (without waiting)
-    and the next sync fails with M2_E_STATE; with the limit restored, a fresh
-    test runs exactly. Direct launches only (graph_iters=1), so no captured
+    and the next sync fails with M2_E_STATE; with the limit restored, the
+    same test set back to its start by a residue, and a fresh test, run
+    exactly. Direct launches only (graph_iters=1), so no captured
     graph keeps the forced value."""
     import ctypes
     c2 = load_fixture("c2.json")["cases"][0]
     prev = ctypes.c_uint32()
     m2.lib().m2_debug_fused_spin_limit(0, ctypes.byref(prev))
+    p = c2["p"]
     try:
-        with m2.Test(gpu_ctx, c2["p"], "prp", graph_iters=1) as t:
+        with m2.Test(gpu_ctx, p, "prp", graph_iters=1) as t:
             t.run(3)
             with pytest.raises(m2.M2Error) as ei:
                 t.sync()
             assert ei.value.code == -6
+            # a test whose state timed out is restored by setting a residue
+            # (here the start, 3, at iteration 0), which clears the report
+            m2.lib().m2_debug_fused_spin_limit(prev.value, None)
+            t.set_residue((3).to_bytes((p + 7) // 8, "little"), 0)
+            t.run(10)
+            assert t.res64() == c2["checkpoints"]["10"]["res64"]
     finally:
         m2.lib().m2_debug_fused_spin_limit(prev.value, None)
-    with m2.Test(gpu_ctx, c2["p"], "prp", graph_iters=1) as t:
+    with m2.Test(gpu_ctx, p, "prp", graph_iters=1) as t:
         t.run(10)
         assert t.res64() == c2["checkpoints"]["10"]["res64"]
 
@@ -345,3 +353,25 @@ def test_server_c5_stream(gpu_ctx, budget_gb):
         with m2.Test(gpu_ctx, p, mode) as t:
             t.run(k)
             assert t.res64() == got[i], (i, p, mode, k)
+
+
+def test_batch_known_mersenne_primes_to_completion(gpu_ctx):
+    """Whole PRP tests of the Mersenne primes 2^p - 1 with p = 216091,
+    756839, 859433, 1257787 and 1398269 (n = 2^13, 2^15, 2^16), a whole LL
+    test of 2^216091 - 1 and a whole PRP test of the composite 2^216103 - 1,
+    all in one batch: the mathematics fixes the primes' final residues (PRP:
+    3^(2^p) = 9 mod 2^p - 1; LL: 0), so every step of these long runs is
+    checked at the end without a fixture."""
+    ps = [216091, 756839, 859433, 1257787, 1398269, 216091, 216103]
+    modes = ["prp"] * 5 + ["ll", "prp"]
+    print("batch of %d whole tests" % len(ps), flush=True)
+    with m2.Batch(gpu_ctx, ps, modes) as b:
+        b.run(0)
+        b.sync()
+        for i, (p, mode) in enumerate(zip(ps, modes)):
+            assert b.iteration(i) == (p if mode == "prp" else p - 2)
+        for i in range(5):
+            assert b.res64(i) == "0000000000000009", ps[i]
+            assert b.is_prime(i)
+        assert b.res64(5) == "0000000000000000" and b.is_prime(5)
+        assert not b.is_prime(6)
