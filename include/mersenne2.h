@@ -215,6 +215,26 @@ int m2_server_step(m2_server *s, int *busy);
 int m2_server_poll(m2_server *s, uint64_t *ids, uint64_t *res64, int *verdicts, size_t max, size_t *count);
 int m2_server_stats(m2_server *s, uint64_t *queued, uint64_t *active, uint64_t *bytes_in_use);
 
+/*
+ * No-alloc variant. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation). Advances the canonical residue
+ * `in` (ceil(p/8) bytes, little-endian) of a test of 2^p - 1 by `iters` PRP
+ * squarings (mode 1) or Lucas-Lehmer steps (mode 0) and writes the result to
+ * `out` (ceil(p/8) bytes): the values m2_test_set_residue + m2_test_run +
+ * m2_test_residue give. Every device buffer it uses (test descriptors, twiddle
+ * tables, digits) lies in one fixed array of M2_NOALLOC_SCRATCH_KIB KiB per
+ * device, set up by m2_open and held until the process ends; the call itself
+ * makes no hipMalloc, hipFree or other allocation. Only exponents whose
+ * transform fits one workgroup (n <= 2^12, p <= 159739) fit the array; larger
+ * ones return M2_E_RANGE. Calls on one device share its array: they are
+ * serialised by a lock, and calls that overlap on one device from several
+ * processes would race. Synchronous: returns when `out` holds the result.
+ */
+#define M2_NOALLOC_SCRATCH_KIB 128
+int m2_square_noalloc(m2_ctx *ctx, uint64_t p, int mode, const uint8_t *in, size_t in_len, uint64_t iters,
+                      uint8_t *out, size_t out_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,6 +73,8 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
                                   _c.c_size_t, _c.POINTER(_c.c_size_t)]),
     "m2_server_stats": (_c.c_int, [_P, _c.POINTER(_c.c_uint64), _c.POINTER(_c.c_uint64),
                                    _c.POINTER(_c.c_uint64)]),
+    "m2_square_noalloc": (_c.c_int, [_P, _c.c_uint64, _c.c_int, _c.c_char_p, _c.c_size_t, _c.c_uint64,
+                                     _c.c_char_p, _c.c_size_t]),
 }
 
 _LIB = None
@@ -431,6 +433,16 @@ class Server:
         q, a, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         _chk(lib().m2_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(b)), "m2_server_stats")
         return {"queued": q.value, "active": a.value, "bytes_in_use": b.value}
+
+
+def square_noalloc(ctx: Context, p: int, mode: str, residue: bytes, iters: int) -> bytes:
+    """The residue after `iters` iterations from `residue`, through the entry
+    point that allocates nothing (m2_square_noalloc; n <= 2^12)."""
+    n = (int(p) + 7) // 8
+    out = ctypes.create_string_buffer(n)
+    _chk(lib().m2_square_noalloc(ctx.h, int(p), _mode(mode), bytes(residue), len(residue), int(iters), out, n),
+         "m2_square_noalloc")
+    return out.raw[:n]
 
 
 def run_checkpoints(ctx: Context, p: int, mode: str, checkpoints: Sequence[int]) -> Dict[int, str]:

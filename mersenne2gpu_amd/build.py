@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 ARCH = os.environ.get("M2_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["m2_kernels.hip", "m2_host.cpp", "m2_api.cpp", "m2_gerbicz.cpp"]
+HIP_SOURCES = ["m2_kernels.hip", "m2_host.cpp", "m2_api.cpp", "m2_gerbicz.cpp", "m2_noalloc.hip"]
 CLI_SOURCES = ["mersenne2_cli.cpp"]
 
 

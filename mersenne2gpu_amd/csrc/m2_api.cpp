@@ -530,6 +530,7 @@ struct m2_batch {
 
 namespace m2 {
 int test_error(int code, const std::string& msg) { return fail(code, msg); }
+int ctx_device(const m2_ctx* ctx) { return ctx->device; }
 int test_params(const m2_test* t, m2_ctx** ctx, uint64_t* p, int* mode) {
   if (!t) return fail(M2_E_INVAL, "null test");
   *ctx = t->ctx;
@@ -563,6 +564,9 @@ int m2_open(int device, m2_ctx** out) {
   ctx->device = device;
   HIPCHK(hipGetDeviceProperties(&ctx->prop, device));
   HIPCHK(hipSetDevice(device));
+  // set-up of m2_square_noalloc on this device; a failure here is reported
+  // by that entry point (which retries it), not by m2_open
+  (void)noalloc_init(device);
   *out = ctx.release();
   return M2_OK;
 }

@@ -120,7 +120,13 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
 hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
                                hipEvent_t start, hipEvent_t stop);
 
-// ---- engine internals shared with m2_gerbicz.cpp (m2_api.cpp) ----
+// ---- engine internals shared with m2_gerbicz.cpp and m2_noalloc.hip (m2_api.cpp) ----
+// Device index of a context.
+int ctx_device(const ::m2_ctx* ctx);
+// Set-up of the no-alloc entry point on `device` (the current device): the
+// scratch array's address, a stream, the single-workgroup tables (once per
+// process). Called by m2_open; m2_noalloc.hip.
+int noalloc_init(int device);
 // Records `msg` as the calling thread's last error and returns `code`.
 int test_error(int code, const std::string& msg);
 // Context, exponent and mode of a test.
