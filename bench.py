@@ -298,6 +298,16 @@ def served_leg(ctx, seed: int, count: int, iters: int, budget_gb: float, slice_i
     return out
 
 
+def _profile_source(name):
+    """The `_source` note of a committed counter summary (which profile and
+    build its figures come from: they are not measured by this run)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            return json.load(f).get("_source")
+    except (OSError, ValueError):
+        return None
+
+
 def _fixture(name):
     path = os.path.join(ROOT, "tests", "golden", name)
     if not os.path.exists(path):
@@ -427,11 +437,7 @@ def main() -> int:
         cpu = cpu_baseline(p, mode, cpu_iters[0], cpu_iters[1], gpu_res64)
         log("cpu baseline:", cpu)
     ctx.close()
-    traffic_src = None
-    if traffic is not None:
-        tj = os.path.join(ROOT, "profiles", "traffic.json")
-        with open(tj) as f:
-            traffic_src = json.load(f).get("_source")
+    traffic_src = _profile_source("traffic.json") if traffic is not None else None
 
     out = {
         "metric": METRIC,
@@ -463,8 +469,7 @@ def main() -> int:
         "int_roofline": {"modmuls_per_iter": int(modmuls),
                          "achieved_gmodmul_s": round(modmuls / (us_per_iter * 1e-6) / 1e9, 1),
                          "kernel": dom, "valu_busy": load_valu(dom, wl["traffic_key"]),
-                         "valu_busy_source": "profiles/valu.json: rocprofv3 SQ counters, VALU instructions "
-                                             "per SIMD per 4-cycle issue slot"},
+                         "valu_busy_source": _profile_source("valu.json")},
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,
