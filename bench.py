@@ -13,8 +13,10 @@ Prints one JSON line. Extra fields:
                 duration (HIP events on the test's stream) against 8 TB/s;
                 traffic = measured HBM bytes per launch from the committed
                 rocprofv3 counter summary named in traffic_source;
-  int_roofline  the integer co-bound: algorithmic base products per iteration
-                (49.5 h per field) per second, and the dominant kernel's VALU
+  int_roofline  the integer co-bound: the iteration's algorithmic base products
+                (49.5 h per field; 4 v_mad_u64_u32 per M61 product, 1 per M31
+                product) per second against the device's v_mad_u64_u32 issue
+                rate measured in the same run, and the dominant kernel's VALU
                 busy fraction per SIMD from the committed counter summary;
   batched       the metric's batched half: 512 PRP tests (n = 2^8..2^16,
                 seed 1) for 2000 iterations each in one m2_batch, tests/s
@@ -423,6 +425,13 @@ def main() -> int:
     # products each at radix-4 counting, 3 base products apiece, plus the
     # centre's 6 h), against the measured time
     modmuls = 2 * 49.5 * h
+    # each M61 product is four 32x32->64 multiply-adds (v_mad_u64_u32), each
+    # M31 product one: the integer roofline prices them at the device's
+    # measured v_mad_u64_u32 issue rate
+    products = 49.5 * h * (4 + 1)
+    mad_rate = ctx.product_rate()
+    int_us = products / mad_rate * 1e6
+    log("v_mad_u64_u32 rate %.2f T lane-op/s -> product floor %.2f us/iter" % (mad_rate / 1e12, int_us))
     t.close()
 
     def gpu_res64(k):
@@ -466,7 +475,12 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
                      "bytes_per_launch": bytes_dom, "us_per_launch": round(per_kernel_us[dom], 3)},
-        "int_roofline": {"modmuls_per_iter": int(modmuls),
+        "int_roofline": {"bound": "v_mad_u64_u32 issue", "unit": "T products/s",
+                         "achieved": round(products / (us_per_iter * 1e-6) / 1e12, 3),
+                         "peak": round(mad_rate / 1e12, 3), "peak_how": "measured: m2_debug_product_rate",
+                         "frac": round(int_us / us_per_iter, 4),
+                         "products_per_iter": int(products), "product_floor_us": round(int_us, 3),
+                         "modmuls_per_iter": int(modmuls),
                          "achieved_gmodmul_s": round(modmuls / (us_per_iter * 1e-6) / 1e9, 1),
                          "kernel": dom, "valu_busy": load_valu(dom, wl["traffic_key"]),
                          "valu_busy_source": _profile_source("valu.json")},

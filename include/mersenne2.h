@@ -113,6 +113,13 @@ int m2_test_time(m2_test *t, uint64_t iters, double *ms);
  * is kernel i's mean device time per launch, in milliseconds. */
 int m2_test_kernel_times(m2_test *t, uint64_t iters, double *ms, int max_kernels, int *count, char *names_buf,
                          size_t names_len);
+/* The same run, reporting each kernel's total device time (ms) and launch
+ * count: the per-kernel event profile behind the CLI's --profile-kernels
+ * (gf61_ntt_prp_opencl_inplace_nobitrev_fused_powonly_blockcarry.cpp:645-698,
+ * which records OpenCL profiling events per launch and prints totals). The
+ * test advances by `iters`; launches are not graphed while profiled. */
+int m2_test_kernel_profile(m2_test *t, uint64_t iters, double *total_ms, uint64_t *launches, int max_kernels,
+                           int *count, char *names_buf, size_t names_len);
 /* Human-readable plan (the reference's --debug-plan, mersenne2_gpu.cpp:4767-4776). */
 int m2_test_plan(m2_test *t, char *buf, size_t len);
 
@@ -123,6 +130,11 @@ int m2_test_plan(m2_test *t, char *buf, size_t len);
  * real stall. Applies to launches made afterwards (a test's captured graphs
  * keep the value they were made with). */
 int m2_debug_fused_spin_limit(uint32_t polls, uint32_t *previous);
+/* Measurement hook (no counterpart in the reference): the device's issue rate
+ * of v_mad_u64_u32, the 32x32->64 multiply-add the field products are made
+ * of, in lane operations per second, timed with hipEvents over a short probe
+ * kernel. It is the peak against which the integer roofline is priced. */
+int m2_debug_product_rate(m2_ctx *ctx, double *lane_ops_per_s);
 
 /* PRP run with a Gerbicz error check (the reference has none; its loop,
  * mersenne2_gpu.cpp:4811-4853, trusts every iteration). Advances a PRP test by

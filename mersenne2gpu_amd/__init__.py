@@ -47,8 +47,11 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "m2_test_time": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double)]),
     "m2_test_kernel_times": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double), _c.c_int, _c.POINTER(_c.c_int),
                                         _c.c_char_p, _c.c_size_t]),
+    "m2_test_kernel_profile": (_c.c_int, [_P, _c.c_uint64, _c.POINTER(_c.c_double), _c.POINTER(_c.c_uint64),
+                                          _c.c_int, _c.POINTER(_c.c_int), _c.c_char_p, _c.c_size_t]),
     "m2_test_plan": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
     "m2_debug_fused_spin_limit": (_c.c_int, [_c.c_uint32, _c.POINTER(_c.c_uint32)]),
+    "m2_debug_product_rate": (_c.c_int, [_P, _c.POINTER(_c.c_double)]),
     "m2_test_run_checked": (_c.c_int, [_P, _c.c_uint64, _c.c_uint32, _c.c_uint32, _c.POINTER(_c.c_uint64)]),
     "m2_debug_gerbicz_fault": (_c.c_int, [_c.c_uint64]),
     "m2_checkpoint_save": (_c.c_int, [_P, _c.c_char_p]),
@@ -159,6 +162,13 @@ class Context:
 
     def test(self, p: int, mode: str = "prp", graph_iters: int = 0) -> "Test":
         return Test(self, p, mode, graph_iters)
+
+    def product_rate(self) -> float:
+        """Measured v_mad_u64_u32 issue rate of the device, lane operations per
+        second (m2_debug_product_rate)."""
+        v = ctypes.c_double()
+        _chk(lib().m2_debug_product_rate(self.h, ctypes.byref(v)), "m2_debug_product_rate")
+        return v.value
 
 
 def debug_gerbicz_fault(block: int) -> None:
@@ -296,6 +306,17 @@ class Test:
              "m2_test_kernel_times")
         keys = names.value.decode().split("\n")[: cnt.value]
         return {k: v for k, v in zip(keys, ms[: cnt.value])}
+
+    def kernel_profile(self, iters: int) -> Dict[str, tuple]:
+        """(total device milliseconds, launches) of each kernel over `iters` iterations."""
+        ms = (ctypes.c_double * 16)()
+        nl = (ctypes.c_uint64 * 16)()
+        cnt = ctypes.c_int()
+        names = ctypes.create_string_buffer(1024)
+        _chk(lib().m2_test_kernel_profile(self.h, int(iters), ms, nl, 16, ctypes.byref(cnt), names, 1024),
+             "m2_test_kernel_profile")
+        keys = names.value.decode().split("\n")[: cnt.value]
+        return {k: (ms[i], nl[i]) for i, k in enumerate(keys)}
 
     def plan(self) -> str:
         buf = ctypes.create_string_buffer(4096)

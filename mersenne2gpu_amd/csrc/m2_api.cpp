@@ -753,14 +753,22 @@ int m2_test_time(m2_test* t, uint64_t iters, double* ms) {
   return test_sync(t);
 }
 
-int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels, int* count, char* names_buf,
-                         size_t names_len) {
-  if (!t || !ms || !count) return fail(M2_E_INVAL, "null pointer");
+}  // extern "C"
+
+namespace {
+
+// Runs `iters` iterations with every launch between its own dispatch
+// timestamps, in runs of up to RUN iterations (no graphs): per-kernel total
+// device time and launch count, kernels in order of first launch.
+struct KernelProfile {
+  std::vector<std::string> order;
+  std::map<std::string, std::pair<double, uint64_t>> acc;  // name -> (total ms, launches)
+};
+
+int kernel_profile(m2_test* t, uint64_t iters, KernelProfile& out) {
   HIPCHK(hipSetDevice(t->ctx->device));
   int rc = test_sync(t);
   if (rc) return rc;
-  // Runs of up to RUN iterations, every launch between its own dispatch
-  // timestamps; per-kernel totals and launch counts give the mean per launch.
   constexpr uint64_t RUN = 16;
   constexpr int MAXE = 2 * (int)RUN + 8;
   Events ev_start, ev_stop;
@@ -773,8 +781,6 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
   if ((rc = t->group.upload_descs({t->obj}, &one, t->group.transient, &d))) return rc;
   const DevPlan& P = t->obj->cls->P;
   const bool fused = t->group.use_fused(1);
-  std::vector<std::string> order;
-  std::map<std::string, std::pair<double, uint64_t>> acc;
   for (uint64_t done = 0; done < iters;) {
     const char* names[MAXE] = {nullptr};
     int nk = 0;
@@ -796,18 +802,48 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
       float f = 0;
       HIPCHK(hipEventElapsedTime(&f, starts[j], stops[j]));
       const std::string nm = names[j] ? names[j] : "?";
-      if (!acc.count(nm)) order.push_back(nm);
-      acc[nm].first += f;
-      acc[nm].second += 1;
+      if (!out.acc.count(nm)) out.order.push_back(nm);
+      out.acc[nm].first += f;
+      out.acc[nm].second += 1;
     }
   }
-  if ((rc = test_sync(t))) return rc;
-  const int m = std::min((int)order.size(), max_kernels);
+  return test_sync(t);
+}
+
+}  // namespace
+
+extern "C" {
+
+int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels, int* count, char* names_buf,
+                         size_t names_len) {
+  if (!t || !ms || !count) return fail(M2_E_INVAL, "null pointer");
+  KernelProfile kp;
+  int rc = kernel_profile(t, iters, kp);
+  if (rc) return rc;
+  const int m = std::min((int)kp.order.size(), max_kernels);
   std::string nm;
   for (int k = 0; k < m; ++k) {
-    const auto& a = acc[order[k]];
+    const auto& a = kp.acc[kp.order[k]];
     ms[k] = a.first / (double)a.second;
-    nm += order[k] + "\n";
+    nm += kp.order[k] + "\n";
+  }
+  *count = m;
+  if (names_buf && names_len) snprintf(names_buf, names_len, "%s", nm.c_str());
+  return M2_OK;
+}
+
+int m2_test_kernel_profile(m2_test* t, uint64_t iters, double* total_ms, uint64_t* launches, int max_kernels,
+                           int* count, char* names_buf, size_t names_len) {
+  if (!t || !total_ms || !launches || !count) return fail(M2_E_INVAL, "null pointer");
+  KernelProfile kp;
+  int rc = kernel_profile(t, iters, kp);
+  if (rc) return rc;
+  const int m = std::min((int)kp.order.size(), max_kernels);
+  std::string nm;
+  for (int k = 0; k < m; ++k) {
+    total_ms[k] = kp.acc[kp.order[k]].first;
+    launches[k] = kp.acc[kp.order[k]].second;
+    nm += kp.order[k] + "\n";
   }
   *count = m;
   if (names_buf && names_len) snprintf(names_buf, names_len, "%s", nm.c_str());
@@ -817,6 +853,36 @@ int m2_test_kernel_times(m2_test* t, uint64_t iters, double* ms, int max_kernels
 int m2_debug_fused_spin_limit(uint32_t polls, uint32_t* previous) {
   if (previous) *previous = g_fused_spin_limit;
   g_fused_spin_limit = polls;
+  return M2_OK;
+}
+
+int m2_debug_product_rate(m2_ctx* ctx, double* lane_ops_per_s) {
+  if (!ctx || !lane_ops_per_s) return fail(M2_E_INVAL, "null pointer");
+  HIPCHK(hipSetDevice(ctx->device));
+  // 16 workgroups of 256 per CU; a short launch first (clocks up, code loaded)
+  const uint32_t blocks = 16u * (uint32_t)std::max(1, ctx->prop.multiProcessorCount);
+  uint64_t* out = nullptr;
+  HIPCHK(hipMalloc(&out, (size_t)blocks * 256 * sizeof(uint64_t)));
+  hipStream_t st = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  uint64_t ops = 0;
+  float ms = 0;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  if (e == hipSuccess) e = launch_mad_rate(out, blocks, 256, st, &ops);
+  if (e == hipSuccess) e = hipEventRecord(a, st);
+  if (e == hipSuccess) e = launch_mad_rate(out, blocks, 8192, st, &ops);
+  if (e == hipSuccess) e = hipEventRecord(b, st);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  if (b) (void)hipEventDestroy(b);
+  if (a) (void)hipEventDestroy(a);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(out);
+  if (e != hipSuccess) return fail(M2_E_HIP, std::string("product rate probe: ") + hipGetErrorString(e));
+  if (ms <= 0) return fail(M2_E_HIP, "product rate probe: no elapsed time");
+  *lane_ops_per_s = (double)ops / (ms * 1e-3);
   return M2_OK;
 }
 

@@ -100,6 +100,9 @@ inline uint64_t carry_target_digit(const DevPlan& P, uint64_t g) {
 
 // ---- launchers (m2_kernels.hip); `tests` is a device array of ntests descriptors ----
 hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st);
+// v_mad_u64_u32 issue-rate probe: `blocks` x 256 threads into out[blocks * 256];
+// *lane_ops receives the multiply-adds the launch issues.
+hipError_t launch_mad_rate(uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t st, uint64_t* lane_ops);
 hipError_t launch_iteration(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st);
 // k iterations of one test with the outer passes of consecutive iterations
 // fused: outer_fwd, (centre, outer_fused) x (k - 1), centre, outer_inv.

@@ -1662,4 +1662,29 @@ hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t
   return hipGetLastError();
 }
 
+// Issue-rate probe for v_mad_u64_u32, the 32x32->64 multiply-add every field
+// product is built from: each thread runs eight independent chains
+// x <- lo32(x) * k + x, one instruction per step.
+constexpr int MAD_CHAINS = 8;
+
+__global__ void __launch_bounds__(256) k_mad_rate(uint64_t* out, uint32_t iters, uint32_t k) {
+  uint64_t x[MAD_CHAINS];
+#pragma unroll
+  for (int c = 0; c < MAD_CHAINS; ++c) x[c] = threadIdx.x * 131u + c * 7u + blockIdx.x;
+  for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int c = 0; c < MAD_CHAINS; ++c) x[c] = (uint64_t)(uint32_t)x[c] * k + x[c];
+  }
+  uint64_t s = 0;
+#pragma unroll
+  for (int c = 0; c < MAD_CHAINS; ++c) s ^= x[c];
+  out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+hipError_t launch_mad_rate(uint64_t* out, uint32_t blocks, uint32_t iters, hipStream_t st, uint64_t* lane_ops) {
+  hipLaunchKernelGGL(k_mad_rate, dim3(blocks), dim3(256), 0, st, out, iters, 0x9E3779B9u);
+  *lane_ops = (uint64_t)blocks * 256u * iters * MAD_CHAINS;
+  return hipGetLastError();
+}
+
 }  // namespace m2

@@ -2,13 +2,18 @@
 //           [-LS cap] [-N max_iters] [--show-defaults] [--enable-chunk64] [--debug-plan]
 //           [--debug-stage-selftest] [--debug-cppint-every N] [--debug-cppint-max-iter N]
 //           [--res64-at k1,k2,...] [--dump-residue path] [--checkpoint path] [--resume path]
-//           [--gerbicz block]
+//           [--gerbicz block] [--profile-kernels] [--profile-every N]
 //
 // Drop-in for the reference's command line (mersenne2_gpu.cpp:3618-4960): same
 // grammar, defaults, stdout lines and exit codes, on top of the libm2 C ABI. It
 // adds a final "Res64: <hex>" line (the reference prints no residue), the
 // --res64-at / --dump-residue options, checkpoints, and a Gerbicz error check
 // for PRP runs (--gerbicz: blocks of that many iterations, a check every ten).
+// --profile-kernels / --profile-every N print per-kernel device-time totals
+// the way the reference's power-only prototype does
+// (gf61_ntt_prp_opencl_inplace_nobitrev_fused_powonly_blockcarry.cpp:645-698,
+// :1277-1313): a summary every N iterations (default: the report interval)
+// and a final one.
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -20,6 +25,7 @@
 #include <iomanip>
 #include <iostream>
 #include <limits>
+#include <map>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -32,11 +38,57 @@ const char* USAGE_TAIL =
     " <p> [ll|prp] [report_every] [-d device_id] [-R report_seconds] [-W fft_wg] [-CW carry_wg] [-B carry_pairs] "
     "[-LS local_stage_cap] [-N max_iters] [--show-defaults] [--enable-chunk64] [--debug-plan] "
     "[--debug-stage-selftest] [--debug-cppint-every N] [--debug-cppint-max-iter N] [--res64-at k1,k2,...] "
-    "[--dump-residue path] [--checkpoint path] [--resume path] [--gerbicz block]\n";
+    "[--dump-residue path] [--checkpoint path] [--resume path] [--gerbicz block] [--profile-kernels] "
+    "[--profile-every N]\n";
 
 int die(const char* what, int code) {
   std::cerr << what << " failed with error " << code << ": " << m2_last_error() << std::endl;
   return 1;
+}
+
+// Per-kernel totals in order of first launch.
+struct Profile {
+  std::vector<std::string> order;
+  std::map<std::string, std::pair<double, uint64_t>> acc;  // name -> (total ms, launches)
+};
+
+int profile_run(m2_test* t, uint64_t iters, Profile& pr) {
+  constexpr int MAXK = 16;
+  double ms[MAXK];
+  uint64_t nl[MAXK];
+  int cnt = 0;
+  char names[1024] = {0};
+  int rc = m2_test_kernel_profile(t, iters, ms, nl, MAXK, &cnt, names, sizeof names);
+  if (rc != M2_OK) return rc;
+  std::stringstream ss(names);
+  std::string nm;
+  for (int k = 0; k < cnt && std::getline(ss, nm); ++k) {
+    if (!pr.acc.count(nm)) pr.order.push_back(nm);
+    pr.acc[nm].first += ms[k];
+    pr.acc[nm].second += nl[k];
+  }
+  return M2_OK;
+}
+
+void profile_print(const Profile& pr, const std::string& title) {
+  if (pr.order.empty()) {
+    std::cout << title << ": no event timing data collected.\n";
+    return;
+  }
+  double total = 0;
+  for (const auto& nm : pr.order) total += pr.acc.at(nm).first;
+  if (total <= 0) {
+    std::cout << title << ": events were recorded but their timings are unavailable.\n";
+    return;
+  }
+  std::vector<std::pair<std::string, std::pair<double, uint64_t>>> rows;
+  for (const auto& nm : pr.order) rows.push_back({nm, pr.acc.at(nm)});
+  std::stable_sort(rows.begin(), rows.end(), [](const auto& a, const auto& b) { return a.second.first > b.second.first; });
+  std::cout << title << ":\n";
+  for (const auto& r : rows)
+    std::cout << "  " << r.first << ": " << std::fixed << std::setprecision(3) << r.second.first << " ms total ("
+              << std::setprecision(1) << 100.0 * r.second.first / total << "%, launches=" << r.second.second << ")\n";
+  std::cout << std::flush;
 }
 
 std::string hex16(uint64_t v) {
@@ -64,6 +116,8 @@ int main(int argc, char* argv[]) {
   std::vector<uint64_t> res64_at;
   std::string dump_path, checkpoint_path, resume_path;
   uint32_t gerbicz_block = 0;
+  bool profile_kernels = false;
+  uint32_t profile_every = 0;
 
   auto need = [&](int i, const char* msg) -> bool {
     if (i + 1 >= argc) {
@@ -135,6 +189,11 @@ int main(int argc, char* argv[]) {
         std::cerr << "--gerbicz needs a block length > 0\n";
         return 1;
       }
+    } else if (arg == "--profile-kernels") {
+      profile_kernels = true;
+    } else if (arg == "--profile-every") {
+      if (!need(i, "Missing value after --profile-every")) return 1;
+      profile_every = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
     } else if (!report_every_set) {
       report_every = (uint32_t)std::strtoul(arg.c_str(), nullptr, 10);
       report_every_set = true;
@@ -253,11 +312,16 @@ int main(int argc, char* argv[]) {
   const bool gerbicz = gerbicz_block && mode == M2_MODE_PRP;
   constexpr uint32_t GERBICZ_BLOCKS = 10;
   uint64_t gerbicz_errors = 0;
+  // per-kernel profile: kernels timed by events (the Gerbicz path runs untimed)
+  if (profile_kernels && gerbicz) std::cout << "profile: --gerbicz runs are not profiled\n";
+  const uint64_t prof_every = (profile_kernels && !gerbicz) ? (profile_every ? profile_every : batch) : 0;
+  Profile prof;
   uint64_t iter = start;
   size_t next_cp = 0;
   while (iter < total) {
     uint64_t chunk_end = gerbicz ? std::min<uint64_t>(total, iter + (uint64_t)gerbicz_block * GERBICZ_BLOCKS)
                                  : std::min<uint64_t>(total, (iter / batch + 1) * batch);
+    if (prof_every) chunk_end = std::min<uint64_t>(chunk_end, (iter / prof_every + 1) * prof_every);
     if (next_cp < res64_at.size() && res64_at[next_cp] > iter && res64_at[next_cp] < chunk_end)
       chunk_end = res64_at[next_cp];
     if (gerbicz) {
@@ -265,12 +329,17 @@ int main(int argc, char* argv[]) {
       rc = m2_test_run_checked(t, chunk_end - iter, gerbicz_block, GERBICZ_BLOCKS, &e);
       gerbicz_errors += e;
       if (rc != M2_OK) return die("m2_test_run_checked", rc);
+    } else if (prof_every) {
+      rc = profile_run(t, chunk_end - iter, prof);
+      if (rc != M2_OK) return die("m2_test_kernel_profile", rc);
     } else {
       rc = m2_test_run(t, chunk_end - iter);
       if (rc == M2_OK) rc = m2_test_sync(t);
       if (rc != M2_OK) return die("m2_test_run", rc);
     }
     iter = chunk_end;
+    if (prof_every && (iter % prof_every == 0 || iter == total))
+      profile_print(prof, "Kernel profile summary at iter " + std::to_string(iter));
     while (next_cp < res64_at.size() && res64_at[next_cp] <= iter) {
       if (res64_at[next_cp] == iter) {
         uint64_t r = 0;
@@ -299,6 +368,7 @@ int main(int argc, char* argv[]) {
   std::cout << "2^" << p << " - 1 is "
             << (prime ? (mode == M2_MODE_LL ? "prime" : "a probable prime") : "composite") << ", elapsed "
             << std::fixed << std::setprecision(2) << el << " s, it/s " << std::setprecision(1) << itps << "\n";
+  if (prof_every) profile_print(prof, "Kernel profile summary (final)");
   std::cout << "Res64: " << hex16(r64) << "\n";
   if (!dump_path.empty()) {
     std::vector<uint8_t> buf((p + 7) / 8);

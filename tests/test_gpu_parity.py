@@ -163,3 +163,47 @@ def test_cli_ll_default_mode_and_benchmark_iters():
     assert "benchmark_iters=1000" in r.stdout
     assert "iter 1000/1000 (100.0%)" in r.stdout
     assert "2^20983 - 1 is composite" in r.stdout
+
+
+def _profile_blocks(out):
+    """Kernel profile summaries printed by --profile-kernels: title -> {kernel: (ms, launches)}."""
+    blocks, cur = {}, None
+    for line in out.splitlines():
+        if line.startswith("Kernel profile summary"):
+            cur = blocks.setdefault(line.rstrip(":"), {})
+        elif cur is not None and line.startswith("  ") and " ms total (" in line:
+            name, rest = line.strip().split(": ", 1)
+            ms = float(rest.split(" ms total")[0])
+            launches = int(rest.split("launches=")[1].rstrip(")"))
+            cur[name] = (ms, launches)
+        else:
+            cur = None
+    return blocks
+
+
+@pytest.mark.parametrize("p, iters, every, kernels", [
+    (27899687, 500, 250, {"k_centre": 500}),  # n = 2^20: fused outer pass + centre per iteration
+    (19937, 2000, 1000, {"k_small": 2000}),    # n = 512: single-workgroup plan, one launch per iteration
+])
+def test_cli_profile_kernels(p, iters, every, kernels):
+    r = subprocess.run([m2.CLI_PATH, str(p), "prp", "-N", str(iters), "--profile-kernels", "--profile-every",
+                        str(every), "--res64-at", str(iters)],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    blocks = _profile_blocks(r.stdout)
+    want = ["Kernel profile summary at iter %d" % k for k in range(every, iters + 1, every)]
+    assert list(blocks) == want + ["Kernel profile summary (final)"]
+    final = blocks["Kernel profile summary (final)"]
+    for prefix, launches in kernels.items():
+        got = [v for k, v in final.items() if k.startswith(prefix)]
+        assert got and sum(n for _, n in got) == launches, final
+    assert all(ms > 0 for ms, _ in final.values())
+    # totals only grow from one summary to the next
+    for a, b in zip(want, want[1:] + ["Kernel profile summary (final)"]):
+        assert all(blocks[b][k][1] >= n for k, (_, n) in blocks[a].items())
+    # profiled launches compute the same residue as graphed ones
+    if p == 27899687:
+        c = [c for c in load_fixture("c2.json")["cases"] if c["p"] == p][0]
+        assert "Res64@%d: %s" % (iters, c["checkpoints"][str(iters)]["res64"]) in r.stdout
+    else:
+        assert "Res64@2000: B5DBC55987E2FD16" in r.stdout  # as in the graphed run above
