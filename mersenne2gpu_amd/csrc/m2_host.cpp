@@ -183,7 +183,10 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
     // A single workgroup's transform is bound by latency: radix-4 steps
     // measured faster than radix-8 up to h = 1024 (13.2 vs 15.4 us/iter at
     // n = 512, 17.3 vs 20.1 at n = 2048), radix-8 faster at h = 2048
-    // (25.4 vs 30.5 at n = 4096).
+    // (25.4 vs 30.5 at n = 4096) and h = 4096 (42.3 vs 51.5 at n = 8192).
+    // Single tests of n = 8192 take the multi-pass profile (26.4 us); the
+    // single-workgroup plan pays at that length in batches, where it holds
+    // one CU per test: 0.19 against 0.26 us per test-iteration for 256 tests.
     P.outer = make_pass(0, L, T.tw61, T.tw31, opts.max_radix ? maxr : (L <= 10 ? 4 : 8));
     P.nseg = (uint32_t)(h / SEG_ELEMS);
   } else {

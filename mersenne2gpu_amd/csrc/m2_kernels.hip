@@ -159,7 +159,6 @@ __device__ __forceinline__ void store_carry(CP G, uint32_t i, i128 v) {
 static_assert(PASS_MAX_ELEMS / SEG_ELEMS <= MP_THREADS_NARROW, "one carry segment per thread");
 static_assert((1 << PASS_MAX_LOG_R) - 1 <= MP_THREADS_NARROW && CENTRE_EARLY_TW <= MP_THREADS_NARROW,
               "one staged twiddle per thread");
-static_assert((1 << SMALL_LOG_H) / SEG_ELEMS <= MP_THREADS_NARROW, "one carry segment per thread");
 struct SegDigits {
   int64_t d[2 * SEG_ELEMS];
 };
@@ -888,7 +887,9 @@ __device__ __forceinline__ void stage_range(e61* S61, e31* S31, const e61* __res
 // Small plan: one workgroup holds the whole residue (h <= 2^LOGH) and runs
 // `iters` iterations without leaving LDS. Digits are in natural order. NTH
 // threads, half per field (M61 and M31 waves side by side, as in the
-// multi-pass kernels): 512 for h <= 2^SMALL_LOG_H, or, for h <= 2^8, 128 —
+// multi-pass kernels): 512 for h <= 2^SMALL_LOG_H (at h = 2^12 each thread
+// owns two carry segments and does two butterflies per radix-8 step), or,
+// for h <= 2^8, 128 —
 // one wave per field, which then holds its field's whole transform, so the
 // radix steps and the squaring need no workgroup barrier (step_sync), only
 // the carry, which reads both fields, does; its LDS (17 KB) also lets several
@@ -898,56 +899,107 @@ __device__ __forceinline__ void stage_range(e61* S61, e31* S31, const e61* __res
 constexpr int SMALL_THREADS = 512;
 constexpr int SMALL_WAVE_LOG_H = 8;
 constexpr int SMALL_WAVE_THREADS = 128;
+// Up to h = 2^11 every twiddle of the small plan is staged in LDS (data 54 KB
+// padded, twiddles 72 KB). At h = 2^12 the padded data (108 KB) and the
+// carries (16 KB) leave room for SMALL_LDS_TW twiddles: the tables of the
+// leading steps (radix 8: 511 entries, radix 4: 1023) are staged, while the
+// last step's table and the squaring twiddles are read from global memory,
+// where all tests of the length share them (they stay in L2).
+constexpr int SMALL_LDS_LOG_H = 11;
+constexpr int SMALL_LDS_TW = 1024;
 
 template <int LOGH, int NTH>
 __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                                const e31* __restrict__ tw31, const e61* __restrict__ sq61,
                                                const e31* __restrict__ sq31, Pass ps) {
   constexpr bool WAVE = NTH == SMALL_WAVE_THREADS;
-  static_assert((1 << LOGH) / SEG_ELEMS <= NTH, "one carry segment per thread");
+  constexpr int NSEG = (1 << LOGH) / SEG_ELEMS;
+  constexpr int SPT = NSEG > NTH ? NSEG / NTH : 1;  // carry segments per thread
+  static_assert(NSEG <= SPT * NTH, "every carry segment has a thread");
   static_assert(!WAVE || LOGH <= SMALL_WAVE_LOG_H, "one wave per field holds the transform");
+  constexpr bool GTW = LOGH > SMALL_LDS_LOG_H;  // part of the twiddles read from global memory
+  constexpr int TN = GTW ? SMALL_LDS_TW : (1 << LOGH);
+  constexpr int SN = GTW ? 1 : (1 << (LOGH - 1));
+  // squaring pairs per thread (h/2 pairs over NTH/2 threads per field)
+  constexpr int SQK = (1 << (LOGH - 1)) / (NTH / 2) > 4 ? (1 << (LOGH - 1)) / (NTH / 2) : 4;
   __shared__ e61 L61[padded(1 << LOGH)];
   __shared__ e31 L31[padded(1 << LOGH)];
-  __shared__ e61 T61[1 << LOGH];
-  __shared__ e31 T31[1 << LOGH];
-  __shared__ e61 S61[1 << (LOGH - 1)];
-  __shared__ e31 S31[1 << (LOGH - 1)];
-  __shared__ Carry128 gs[(1 << LOGH) / SEG_ELEMS];
+  __shared__ e61 T61[TN];
+  __shared__ e31 T31[TN];
+  __shared__ e61 S61[SN];
+  __shared__ e31 S31[SN];
+  __shared__ Carry128 gs[NSEG];
   const TestDesc td = tests[blockIdx.x];
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   const DigitGen D = td.dgen;
   const int logH = ps.A;
   const int h = 1 << logH;
   const int nseg = h / SEG_ELEMS;
-  const uint32_t j0[1] = {0};
-  stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
-  stage_range(S61, S31, sq61, sq31, 0, h >> 1);
-  // thread s owns carry segment s (digits 8s..8s+7) for the whole launch:
-  // its digits stay in registers between iterations
-  const int s = threadIdx.x;
-  const bool mine = s < nseg;
-  const int idx0 = s * SEG_ELEMS;
-  const uint32_t j = 2u * (uint32_t)idx0;
-  SegDigits sd;
-  if (mine) seg_load(z61 + idx0, sd);
-  const SegMeta M = seg_meta(D, j);
+  // steps [0, kl) take their twiddles from LDS, the rest (GTW) from global
+  // memory at the same offsets
+  int kl = ps.nsteps;
+  const e61* G61 = tw61 + ps.steps[0].twoff;
+  const e31* G31 = tw31 + ps.steps[0].twoff;
+  if constexpr (GTW) {
+    int lo = 0;
+    kl = 0;
+    for (; kl < ps.nsteps; ++kl) {
+      const int sp = twiddle_span(ps.steps[kl], ps.steps[kl].ell);
+      if (lo + sp > TN) break;
+      lo += sp;
+    }
+    gather2<1>(T61, T31, G61, G31, lo, [](int i) { return (uint32_t)i; }, [](int i) { return i; });
+  } else {
+    const uint32_t j0[1] = {0};
+    stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
+    stage_range(S61, S31, sq61, sq31, 0, h >> 1);
+  }
+  // thread t owns carry segments t SPT .. t SPT + SPT - 1 (segment s: digits
+  // 8s..8s+7) for the whole launch: their digits stay in registers between
+  // iterations
+  const int s0 = threadIdx.x * SPT;
+  SegDigits sd[SPT];
+  SegMeta M[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; ++i) {
+    if (s0 + i < nseg) seg_load(z61 + (s0 + i) * SEG_ELEMS, sd[i]);
+    M[i] = seg_meta(D, 2u * (uint32_t)((s0 + i) * SEG_ELEMS));
+  }
   const WidthMasks wm(D);
   for (uint32_t it = 0; it < td.iters; ++it) {
-    if (mine) seg_weight(sd, L61, L31, idx0, M);
+#pragma unroll
+    for (int i = 0; i < SPT; ++i)
+      if (s0 + i < nseg) seg_weight(sd[i], L61, L31, (s0 + i) * SEG_ELEMS, M[i]);
     __syncthreads();
-    lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31);
-    square_self<true, WAVE>(L61, L31, logH, 1, [&](int, int q) { return S61[q]; },
-                            [&](int, int q) { return S31[q]; });
-    lds_pass<false, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31);
+    lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31, 0, kl);
+    if constexpr (GTW) {
+      lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, G61, G31, kl);
+      square_self<true, WAVE, SQK>(L61, L31, logH, 1, [&](int, int q) { return sq61[q]; },
+                                   [&](int, int q) { return sq31[q]; });
+      lds_pass<false, true, true, WAVE>(L61, L31, 0, 1, ps, G61, G31, 0, ps.nsteps - kl);
+    } else {
+      square_self<true, WAVE, SQK>(L61, L31, logH, 1, [&](int, int q) { return S61[q]; },
+                                   [&](int, int q) { return S31[q]; });
+    }
+    lds_pass<false, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31, ps.nsteps - kl);
     if (WAVE) __syncthreads();  // the carry reads both fields
-    if (mine) {
-      const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
-      store_carry(gs, s, seg_normalise(L61, L31, idx0, M, wm, D.q, c0, td.mode == 0 ? ~0ull : 0ull, sd));
+#pragma unroll
+    for (int i = 0; i < SPT; ++i) {
+      const int s = s0 + i;
+      if (s < nseg) {
+        const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
+        store_carry(gs, s, seg_normalise(L61, L31, s * SEG_ELEMS, M[i], wm, D.q, c0, td.mode == 0 ? ~0ull : 0ull,
+                                         sd[i]));
+      }
     }
     __syncthreads();
-    if (mine) seg_ripple(sd, M, wm, D.q, load_carry(gs, (s + nseg - 1) % nseg));
+#pragma unroll
+    for (int i = 0; i < SPT; ++i)
+      if (s0 + i < nseg) seg_ripple(sd[i], M[i], wm, D.q, load_carry(gs, (s0 + i + nseg - 1) % nseg));
   }
-  if (mine) seg_store(z61 + idx0, sd);
+#pragma unroll
+  for (int i = 0; i < SPT; ++i)
+    if (s0 + i < nseg) seg_store(z61 + (s0 + i) * SEG_ELEMS, sd[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1471,13 +1523,17 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == M
 // Host-side launchers, used by the engine.
 
 hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
-  constexpr int WL = SMALL_WAVE_LOG_H, WT = SMALL_WAVE_THREADS;
+  constexpr int WL = SMALL_WAVE_LOG_H, WT = SMALL_WAVE_THREADS, LL = SMALL_LDS_LOG_H, T = SMALL_THREADS;
+  static_assert(SMALL_LOG_H <= LL + 1, "one kernel for the lengths past the all-LDS ones");
   if (P.L <= WL)
     hipLaunchKernelGGL((k_small<WL, WT>), dim3(ntests), dim3(WT), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31,
                        P.outer);
+  else if (P.L <= LL)
+    hipLaunchKernelGGL((k_small<LL, T>), dim3(ntests), dim3(T), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31,
+                       P.outer);
   else
-    hipLaunchKernelGGL((k_small<SMALL_LOG_H, SMALL_THREADS>), dim3(ntests), dim3(SMALL_THREADS), 0, st, tests, P.tw61,
-                       P.tw31, P.sq61, P.sq31, P.outer);
+    hipLaunchKernelGGL((k_small<SMALL_LOG_H, T>), dim3(ntests), dim3(T), 0, st, tests, P.tw61, P.tw31, P.sq61,
+                       P.sq31, P.outer);
   return hipGetLastError();
 }
 
@@ -1652,13 +1708,16 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
 
 hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
                                hipEvent_t start, hipEvent_t stop) {
-  constexpr int WL = SMALL_WAVE_LOG_H, WT = SMALL_WAVE_THREADS;
+  constexpr int WL = SMALL_WAVE_LOG_H, WT = SMALL_WAVE_THREADS, LL = SMALL_LDS_LOG_H, T = SMALL_THREADS;
   if (P.L <= WL)
     hipExtLaunchKernelGGL((k_small<WL, WT>), dim3(ntests), dim3(WT), 0, st, start, stop, 0, tests, P.tw61, P.tw31,
                           P.sq61, P.sq31, P.outer);
+  else if (P.L <= LL)
+    hipExtLaunchKernelGGL((k_small<LL, T>), dim3(ntests), dim3(T), 0, st, start, stop, 0, tests, P.tw61, P.tw31,
+                          P.sq61, P.sq31, P.outer);
   else
-    hipExtLaunchKernelGGL((k_small<SMALL_LOG_H, SMALL_THREADS>), dim3(ntests), dim3(SMALL_THREADS), 0, st, start, stop,
-                          0, tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
+    hipExtLaunchKernelGGL((k_small<SMALL_LOG_H, T>), dim3(ntests), dim3(T), 0, st, start, stop, 0, tests, P.tw61,
+                          P.tw31, P.sq61, P.sq31, P.outer);
   return hipGetLastError();
 }
 

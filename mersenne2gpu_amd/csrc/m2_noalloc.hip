@@ -28,7 +28,8 @@ namespace m2 {
 // Enough for the largest length (n = 2^12: 2 descriptors, 2047 step twiddles
 // and 1024 centre twiddles in both fields, 2048 digit pairs: 106 KiB).
 constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(M2_NOALLOC_SCRATCH_KIB) * 1024;
-constexpr int NOALLOC_MAX_LN = SMALL_LOG_H + 1;
+constexpr int NOALLOC_MAX_LN = 12;
+static_assert(NOALLOC_MAX_LN <= SMALL_LOG_H + 1, "every length is a single-workgroup plan");
 constexpr int NOALLOC_MAX_DEVICES = 16;
 constexpr uint32_t NOALLOC_CHUNK = 4096;  // iterations per launch (as m2_test's single-workgroup plan)
 

@@ -18,7 +18,7 @@
 namespace m2 {
 
 constexpr int MAX_STEPS = 8;
-constexpr int SMALL_LOG_H = 11;     // h <= 2048 elements (48 KB of data in LDS)
+constexpr int SMALL_LOG_H = 12;     // h <= 4096 elements (96 KB of data in LDS)
 constexpr int CENTRE_LOG_C = 10;    // centre block: 1024 elements, two per workgroup
 constexpr int PASS_MAX_ELEMS = 2048;  // R * W elements per strided-pass workgroup
 constexpr int PASS_MAX_LOG_R = 9;     // strided passes: at most 2^9 rows (R - 1 twiddles)

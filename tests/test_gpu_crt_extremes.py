@@ -44,7 +44,7 @@ def _r64(x):
     return "%016X" % ((x - 1 if x < 0 else x) & ((1 << 64) - 1))
 
 
-def _run_case(ctx, p, mode, d, iters, ls=0):
+def _run_case(ctx, p, mode, d, iters, ls=0, small=False, **opts):
     """Sets R = M_p - d (d small, or a power of two 2^k with p < 2k < 2p),
     runs `iters` iterations one at a time and checks each Res64 against the
     closed form: the first squaring gives d^2 (2^(2k - p) for d = 2^k), every
@@ -54,7 +54,8 @@ def _run_case(ctx, p, mode, d, iters, ls=0):
     c = 2 if mode == "ll" else 0
     k = d.bit_length() - 1
     v = (1 << (2 * k - p)) if d == 1 << k and 2 * k > p else d * d
-    with m2.Test(ctx, p, mode, local_stage_cap=ls) as t:
+    with m2.Test(ctx, p, mode, local_stage_cap=ls, **opts) as t:
+        assert ("k_small" in t.plan()) == small or not (small or opts), t.plan()
         t.set_residue(((1 << p) - 1 - d).to_bytes((p + 7) // 8, "little"), 0)
         for it in range(1, iters + 1):
             t.run(1)
@@ -100,3 +101,16 @@ def test_lucas_lehmer_end_of_a_prime_test(gpu_ctx):
         assert s3 in ((1 << ((p + 1) // 2)) % M, M - (1 << ((p + 1) // 2)))
         t.run(1)
         assert t.res64() == "0000000000000000" and t.is_prime()
+
+
+@pytest.mark.parametrize("radix", [8, 4])
+def test_near_maximal_residues_on_lds_resident_n2p13(gpu_ctx, radix):
+    """n = 2^13 on the single-workgroup plan (h = 2^12: two carry segments per
+    thread, the last step's and the squaring twiddles read from global
+    memory), which a plain single test of that length does not use: an
+    explicit radix selects the default plan instead of the single-test
+    profile."""
+    for p in (TOPS[13], 159763):
+        _run_case(gpu_ctx, p, "prp", 3, 3, small=True, max_radix=radix)
+        _run_case(gpu_ctx, p, "ll", 1, 3, small=True, max_radix=radix)
+        _run_case(gpu_ctx, p, "ll", 1 << ((p + 1) // 2), 1, small=True, max_radix=radix)
