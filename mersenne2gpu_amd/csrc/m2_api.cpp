@@ -126,7 +126,6 @@ struct TestObj {
   uint64_t iter = 0;    // iterations done
   uint64_t target = 0;  // iterations wanted (batch / server)
   DigitGen dgen{};
-  TestTables tt;
   TestDesc desc{};
   std::vector<void*> allocs;
   uint64_t bytes = 0;
@@ -250,7 +249,7 @@ int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, const PlanOpts& po, hipSt
   t->mode = mode;
   t->target = full_iters(t->p, mode);
   std::string err;
-  if (!build_test_tables(t->p, cls->P, t->tt, t->dgen, err)) return fail(M2_E_RANGE, err);
+  if (!make_digit_gen(t->p, cls->P, t->dgen, err)) return fail(M2_E_RANGE, err);
   const DevPlan& P = cls->P;
   const uint64_t h = P.h;
   void* d;
@@ -281,10 +280,17 @@ int test_obj_create(m2_ctx* ctx, uint64_t p, int mode, const PlanOpts& po, hipSt
   t->desc.dgen = t->dgen;
   t->desc.mode = mode;
   t->desc.iters = 0;
-  // initial residue: 3 (PRP) or 4 (LL) in digit 0 (mersenne2_gpu.cpp:3908)
-  std::vector<int64_t> digits(2 * h, 0);
-  digits[0] = (mode == M2_MODE_LL) ? 4 : 3;
-  if ((rc = upload_digits(t.get(), digits.data(), st))) return bail(rc);
+  // initial residue: 3 (PRP) or 4 (LL) in digit 0 (mersenne2_gpu.cpp:3908),
+  // every other digit and pending carry 0: cleared on the device, and the
+  // element holding digit 0 written
+  {
+    e61* z = P.small ? t->desc.z61 : t->desc.zd;
+    const e61 first{(uint64_t)(mode == M2_MODE_LL ? 4 : 3), 0};
+    HIPCHK(hipMemsetAsync(z, 0, h * sizeof(e61), st));
+    HIPCHK(hipMemcpyAsync(z + digit_slot(P, 0), &first, sizeof first, hipMemcpyHostToDevice, st));
+    if (!P.small) HIPCHK(hipMemsetAsync(t->desc.G, 0, P.nseg * sizeof(Carry128), st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
   *out = t.release();
   return M2_OK;
 }
@@ -313,7 +319,7 @@ int test_obj_residue(TestObj* t, std::vector<uint8_t>& r) {
     at.resize(G.size());
     for (uint64_t g = 0; g < G.size(); ++g) at[g] = carry_target_digit(P, g);
   }
-  r = residue_from_state(t->p, t->tt.bitpos, digits.data(), n, G.data(), at.data(), G.size());
+  r = residue_from_state(t->p, P.ln, digits.data(), n, G.data(), at.data(), G.size());
   return M2_OK;
 }
 
@@ -729,7 +735,7 @@ int m2_test_set_residue(m2_test* t, const uint8_t* buf, size_t len, uint64_t ite
   int rc = t->group.sync();
   if (rc) return rc;
   std::vector<int64_t> digits(2ull * t->obj->cls->P.h);
-  digits_from_residue(t->obj->p, t->obj->tt.bitpos, t->obj->tt.dig, buf, len, digits.data());
+  digits_from_residue(t->obj->p, t->obj->cls->P.ln, buf, len, digits.data());
   if ((rc = upload_digits(t->obj, digits.data(), t->group.stream))) return rc;
   t->obj->iter = iteration;
   return M2_OK;

@@ -295,7 +295,7 @@ bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, 
   return true;
 }
 
-bool build_test_tables(uint32_t p, const DevPlan& P, TestTables& T, DigitGen& D, std::string& err) {
+bool make_digit_gen(uint32_t p, const DevPlan& P, DigitGen& D, std::string& err) {
   const int ln = P.ln;
   if (transform_log2(p) != ln) {
     err = "exponent does not match the plan's transform length";
@@ -315,24 +315,8 @@ bool build_test_tables(uint32_t p, const DevPlan& P, TestTables& T, DigitGen& D,
   // Digit widths ceil(p(j+1)/n) - ceil(pj/n) and weight shifts
   // lr*(n - (pj mod n)) with 2^lr an n-th root of 2 in each field
   // (mersenne2_gpu.cpp:3921-3944).
-  T.dig.assign(n, DigitInfo{0, 0, 0, 0});
-  T.bitpos.assign(n + 1, 0);
   const uint64_t lr61 = ((uint64_t(1) << 60) >> ln) % 61u;
   const uint64_t lr31 = ((uint64_t(1) << 30) >> ln) % 31u;
-  for (uint64_t j = 0; j <= n; ++j) T.bitpos[j] = j ? (uint32_t)(((uint64_t)p * j - 1) / n + 1) : 0u;
-  for (uint64_t j = 0; j < n; ++j) {
-    const uint32_t wd = T.bitpos[j + 1] - T.bitpos[j];
-    if (wd != q && wd != q + 1) {
-      err = "digit width generation error";
-      return false;
-    }
-    T.dig[j].width = (uint8_t)wd;
-    if (j) {
-      const uint64_t r = ((uint64_t)p * j) & (n - 1);
-      T.dig[j].w61 = (uint8_t)((lr61 * (n - r)) % 61u);
-      T.dig[j].w31 = (uint8_t)((lr31 * (n - r)) % 31u);
-    }
-  }
   D = DigitGen{};
   D.p = p;
   D.nmask = (uint32_t)(n - 1);
@@ -344,9 +328,11 @@ bool build_test_tables(uint32_t p, const DevPlan& P, TestTables& T, DigitGen& D,
   D.d31 = (uint32_t)((lr31 * D.s) % 31u);
   D.k61 = (uint32_t)((ln + 1) % 61);
   D.k31 = (uint32_t)((ln + 1) % 31);
-  // The kernels walk widths and weights incrementally; check the walk against
-  // the tables above for every digit.
+  // The kernels walk widths and weights incrementally; check the walk
+  // against the closed forms for every digit (and its restarts every 4096).
   DWalk w = dwalk_at(D, 0);
+  uint64_t pj = 0;  // p j
+  uint64_t b0 = 0;  // ceil(p j / n)
   for (uint64_t j = 0; j < n; ++j) {
     if (j % 4096 == 0) {
       const DWalk f = dwalk_at(D, (uint32_t)j);
@@ -355,11 +341,18 @@ bool build_test_tables(uint32_t p, const DevPlan& P, TestTables& T, DigitGen& D,
         return false;
       }
     }
-    if (dwalk_width(D, w) != T.dig[j].width || dwalk_w61(w) != T.dig[j].w61 || dwalk_w31(w) != T.dig[j].w31) {
+    const uint64_t b1 = ((pj + p - 1) >> ln) + 1;
+    const uint32_t wd = (uint32_t)(b1 - b0);
+    const uint64_t r = pj & (n - 1);
+    const uint32_t w61 = j ? (uint32_t)((lr61 * (n - r)) % 61u) : 0u;
+    const uint32_t w31 = j ? (uint32_t)((lr31 * (n - r)) % 31u) : 0u;
+    if ((wd != q && wd != q + 1) || dwalk_width(D, w) != wd || dwalk_w61(w) != w61 || dwalk_w31(w) != w31) {
       err = "digit walk mismatch at digit " + std::to_string(j);
       return false;
     }
     dwalk_next(D, w);
+    pj += p;
+    b0 = b1;
   }
   return true;
 }
@@ -476,20 +469,19 @@ struct Big {
 
 }  // namespace
 
-std::vector<uint8_t> residue_from_state(uint32_t p, const std::vector<uint32_t>& bitpos, const int64_t* digits,
-                                        uint64_t ndigits, const Carry128* carries, const uint64_t* carry_digit,
-                                        uint64_t ncarries) {
+std::vector<uint8_t> residue_from_state(uint32_t p, int ln, const int64_t* digits, uint64_t ndigits,
+                                        const Carry128* carries, const uint64_t* carry_digit, uint64_t ncarries) {
   const size_t nw = p / 64 + 4;
   Big pos(nw), neg(nw);
   for (uint64_t j = 0; j < ndigits; ++j) {
     const int64_t d = digits[j];
-    if (d > 0) pos.add_at(bitpos[j], (unsigned __int128)(uint64_t)d);
-    if (d < 0) neg.add_at(bitpos[j], (unsigned __int128)(uint64_t)(-(d + 1)) + 1u);
+    if (d > 0) pos.add_at(digit_bitpos(p, ln, j), (unsigned __int128)(uint64_t)d);
+    if (d < 0) neg.add_at(digit_bitpos(p, ln, j), (unsigned __int128)(uint64_t)(-(d + 1)) + 1u);
   }
   for (uint64_t s = 0; s < ncarries; ++s) {
     const __int128 v = (__int128)(((unsigned __int128)(uint64_t)carries[s].hi << 64) | carries[s].lo);
     if (v == 0) continue;
-    const uint64_t b = bitpos[carry_digit[s]];
+    const uint64_t b = digit_bitpos(p, ln, carry_digit[s]);
     if (v > 0) pos.add_at(b, (unsigned __int128)v);
     else neg.add_at(b, (unsigned __int128)(-(v + 1)) + 1u);
   }
@@ -517,13 +509,13 @@ std::vector<uint8_t> residue_from_state(uint32_t p, const std::vector<uint32_t>&
   return out;
 }
 
-void digits_from_residue(uint32_t p, const std::vector<uint32_t>& bitpos, const std::vector<DigitInfo>& dig,
-                         const uint8_t* r, size_t len, int64_t* digits) {
-  (void)p;
+void digits_from_residue(uint32_t p, int ln, const uint8_t* r, size_t len, int64_t* digits) {
   auto bit = [&](uint64_t b) -> uint64_t { return (b / 8 < len) ? ((r[b / 8] >> (b % 8)) & 1u) : 0u; };
-  for (size_t j = 0; j < dig.size(); ++j) {
+  const uint64_t n = uint64_t(1) << ln;
+  for (uint64_t j = 0; j < n; ++j) {
+    const uint64_t b0 = digit_bitpos(p, ln, j), wd = digit_bitpos(p, ln, j + 1) - b0;
     uint64_t v = 0;
-    for (unsigned k = 0; k < dig[j].width; ++k) v |= bit((uint64_t)bitpos[j] + k) << k;
+    for (unsigned k = 0; k < wd; ++k) v |= bit(b0 + k) << k;
     digits[j] = (int64_t)v;
   }
 }

@@ -146,10 +146,6 @@ struct ClassTables {
   std::vector<e61> ctw61;  // multi-pass plan: per-workgroup centre records
   std::vector<e31> ctw31;
 };
-struct TestTables {
-  std::vector<DigitInfo> dig;    // n
-  std::vector<uint32_t> bitpos;  // n + 1: ceil(p j / n)
-};
 
 // Plan choices for one transform length; zero fields mean automatic.
 //   ls_cap      (-LS) the centre block is the largest power of two <= ls_cap
@@ -173,18 +169,22 @@ struct PlanOpts {
 
 // Plan and twiddle tables for transform length 2^ln.
 bool build_class_plan(int ln, const PlanOpts& opts, DevPlan& P, ClassTables& T, std::string& err);
-// Digit tables and walk constants of exponent p on plan P (checks the walk
-// against the tables for every digit, and the carry bound of the plan).
-bool build_test_tables(uint32_t p, const DevPlan& P, TestTables& T, DigitGen& D, std::string& err);
+// Walk constants of exponent p on plan P. Checks the kernels' incremental
+// walk against the closed forms (width, weight shifts) at every digit, and
+// the carry bound of the plan.
+bool make_digit_gen(uint32_t p, const DevPlan& P, DigitGen& D, std::string& err);
+// Bit position of digit j of exponent p on a length-2^ln transform:
+// ceil(p j / 2^ln) (digit j has width digit_bitpos(j + 1) - digit_bitpos(j)).
+inline uint64_t digit_bitpos(uint32_t p, int ln, uint64_t j) {
+  return j ? ((((uint64_t)p * j) - 1) >> ln) + 1 : 0;
+}
 std::string describe_plan(const DevPlan& P);
 
 // Canonical residue (little-endian, ceil(p/8) bytes) of the state given by
 // signed digits plus carries pending at given digit indices.
-std::vector<uint8_t> residue_from_state(uint32_t p, const std::vector<uint32_t>& bitpos, const int64_t* digits,
-                                        uint64_t ndigits, const Carry128* carries, const uint64_t* carry_digit,
-                                        uint64_t ncarries);
-// Digits (int64) of a canonical residue.
-void digits_from_residue(uint32_t p, const std::vector<uint32_t>& bitpos, const std::vector<DigitInfo>& dig,
-                         const uint8_t* r, size_t len, int64_t* digits);
+std::vector<uint8_t> residue_from_state(uint32_t p, int ln, const int64_t* digits, uint64_t ndigits,
+                                        const Carry128* carries, const uint64_t* carry_digit, uint64_t ncarries);
+// The 2^ln digits (int64) of a canonical residue.
+void digits_from_residue(uint32_t p, int ln, const uint8_t* r, size_t len, int64_t* digits);
 
 }  // namespace m2

@@ -33,7 +33,6 @@ int main() {
     for (uint32_t cap : caps) for (const auto& var : variants) {
       DevPlan P;
       ClassTables CT;
-      TestTables T;
       DigitGen D;
       std::string err;
       PlanOpts po;
@@ -45,7 +44,7 @@ int main() {
       CHECK(okc, "p=%u cap=%u class plan: %s", p, cap, err.c_str());
       if (!okc) continue;
       CHECK(CT.sq61.size() == P.h / 2 && CT.sq31.size() == P.h / 2, "p=%u centre table size", p);
-      const bool ok = build_test_tables(p, P, T, D, err);
+      const bool ok = make_digit_gen(p, P, D, err);
       if (!ok) {
         // only the multi-pass carry bound (q >= 12) may refuse a plan
         CHECK(err.find("12 bits") != std::string::npos, "p=%u cap=%u: %s", p, cap, err.c_str());
@@ -83,26 +82,27 @@ int main() {
         CHECK(P.centre_nt == MP_THREADS_WIDE, "p=%u centre size", p);
       }
       if (cap == 0 && var[0] == 0 && var[1] == 8 && var[2] == 0 && n <= (1u << 16)) {
+        auto width = [&](uint64_t j) { return digit_bitpos(p, P.ln, j + 1) - digit_bitpos(p, P.ln, j); };
         std::vector<int64_t> d(n);
-        for (uint64_t j = 0; j < n; ++j) d[j] = (int64_t)(rng() % (1ull << T.dig[j].width));
-        std::vector<uint8_t> r = residue_from_state(p, T.bitpos, d.data(), n, nullptr, nullptr, 0);
+        for (uint64_t j = 0; j < n; ++j) d[j] = (int64_t)(rng() % (1ull << width(j)));
+        std::vector<uint8_t> r = residue_from_state(p, P.ln, d.data(), n, nullptr, nullptr, 0);
         std::vector<int64_t> d2(n);
-        digits_from_residue(p, T.bitpos, T.dig, r.data(), r.size(), d2.data());
-        std::vector<uint8_t> r2 = residue_from_state(p, T.bitpos, d2.data(), n, nullptr, nullptr, 0);
+        digits_from_residue(p, P.ln, r.data(), r.size(), d2.data());
+        std::vector<uint8_t> r2 = residue_from_state(p, P.ln, d2.data(), n, nullptr, nullptr, 0);
         CHECK(r == r2, "p=%u residue round trip", p);
         // moving one unit between digit j and the next is value-neutral:
         // d_j += 2^w_j, d_{j+1} -= 1  (here on j = 1, so digit 2 may go negative)
         std::vector<int64_t> d3 = d2;
-        d3[1] += (int64_t)1 << T.dig[1].width;
+        d3[1] += (int64_t)1 << width(1);
         d3[2] -= 1;
-        CHECK(residue_from_state(p, T.bitpos, d3.data(), n, nullptr, nullptr, 0) == r, "p=%u carry move", p);
+        CHECK(residue_from_state(p, P.ln, d3.data(), n, nullptr, nullptr, 0) == r, "p=%u carry move", p);
         // a pending carry at digit 0 of 1 equals adding 1 to digit 0
         Carry128 c{1, 0};
         uint64_t at = 0;
         std::vector<int64_t> d4 = d2;
         d4[0] += 1;
-        CHECK(residue_from_state(p, T.bitpos, d2.data(), n, &c, &at, 1) ==
-                  residue_from_state(p, T.bitpos, d4.data(), n, nullptr, nullptr, 0),
+        CHECK(residue_from_state(p, P.ln, d2.data(), n, &c, &at, 1) ==
+                  residue_from_state(p, P.ln, d4.data(), n, nullptr, nullptr, 0),
               "p=%u pending carry", p);
       }
     }
