@@ -304,7 +304,16 @@ void test_obj_destroy(TestObj* t) {
 int test_obj_residue(TestObj* t, std::vector<uint8_t>& r) {
   const DevPlan& P = t->cls->P;
   const uint64_t h = P.h, n = 2 * h;
-  std::vector<int64_t> slots(n), digits(n);
+  // staging kept per thread up to n = 2^22 (64 MiB): a server reads one
+  // residue after another, and fresh pages for every read cost more than the
+  // conversion itself at that length
+  static thread_local std::vector<int64_t> keep_slots, keep_digits;
+  std::vector<int64_t> own_slots, own_digits;
+  const bool keep = n <= (uint64_t(1) << 22);
+  std::vector<int64_t>& slots = keep ? keep_slots : own_slots;
+  std::vector<int64_t>& digits = keep ? keep_digits : own_digits;
+  if (slots.size() < n) slots.resize(n);
+  if (digits.size() < n) digits.resize(n);
   HIPCHK(hipMemcpy(slots.data(), P.small ? t->desc.z61 : t->desc.zd, h * sizeof(e61), hipMemcpyDeviceToHost));
   for (uint64_t e = 0; e < h; ++e) {
     const uint64_t s = digit_slot(P, e);

@@ -472,22 +472,54 @@ struct Big {
 std::vector<uint8_t> residue_from_state(uint32_t p, int ln, const int64_t* digits, uint64_t ndigits,
                                         const Carry128* carries, const uint64_t* carry_digit, uint64_t ncarries) {
   const size_t nw = p / 64 + 4;
-  Big pos(nw), neg(nw);
-  for (uint64_t j = 0; j < ndigits; ++j) {
-    const int64_t d = digits[j];
-    if (d > 0) pos.add_at(digit_bitpos(p, ln, j), (unsigned __int128)(uint64_t)d);
-    if (d < 0) neg.add_at(digit_bitpos(p, ln, j), (unsigned __int128)(uint64_t)(-(d + 1)) + 1u);
+  Big pos(nw), neg(0);  // neg is sized on its first term
+  auto neg_at = [&](uint64_t bit, unsigned __int128 v) {
+    if (neg.w.empty()) neg.w.assign(nw, 0);
+    neg.add_at(bit, v);
+  };
+  // Digits that are all non-negative and below 2^62 (every state the kernels
+  // leave) are summed in one streaming pass: a 128-bit accumulator holds the
+  // bits from word `wi` on, and each digit is added at its bit offset in it.
+  bool stream = true;
+  for (uint64_t j = 0; j < ndigits && stream; ++j) stream = digits[j] >= 0 && (digits[j] >> 62) == 0;
+  if (stream) {
+    // Invariant: digit j's offset in the accumulator, ceil(p j / n) - 64 wi,
+    // is below 64; a digit adds at most 47 bits, so after it at most one
+    // word is complete. Branch-free: the current word is always written
+    // and the accumulator moves on when it is complete.
+    unsigned __int128 acc = 0;
+    uint64_t wi = 0, pj = 0;
+    const uint64_t n1 = (uint64_t(1) << ln) - 1;
+    for (uint64_t j = 0; j < ndigits; ++j) {
+      const uint64_t d = (uint64_t)digits[j];
+      const uint64_t sh = ((pj + n1) >> ln) - 64 * wi;
+      acc += ((unsigned __int128)((d >> 1) >> (63 - sh)) << 64) | (d << sh);
+      pj += p;
+      pos.w[wi] = (uint64_t)acc;
+      const uint64_t next = (((pj + n1) >> ln) - 64 * wi) >> 6;  // 0 or 1
+      acc = next ? (acc >> 64) : acc;
+      wi += next;
+    }
+    for (; acc; acc >>= 64) pos.w[wi++] = (uint64_t)acc;
+  } else {
+    for (uint64_t j = 0; j < ndigits; ++j) {
+      const int64_t d = digits[j];
+      if (d > 0) pos.add_at(digit_bitpos(p, ln, j), (unsigned __int128)(uint64_t)d);
+      if (d < 0) neg_at(digit_bitpos(p, ln, j), (unsigned __int128)(uint64_t)(-(d + 1)) + 1u);
+    }
   }
   for (uint64_t s = 0; s < ncarries; ++s) {
     const __int128 v = (__int128)(((unsigned __int128)(uint64_t)carries[s].hi << 64) | carries[s].lo);
     if (v == 0) continue;
     const uint64_t b = digit_bitpos(p, ln, carry_digit[s]);
     if (v > 0) pos.add_at(b, (unsigned __int128)v);
-    else neg.add_at(b, (unsigned __int128)(-(v + 1)) + 1u);
+    else neg_at(b, (unsigned __int128)(-(v + 1)) + 1u);
   }
   pos.mod_mersenne(p);
-  neg.mod_mersenne(p);
-  if (!pos.ge(neg)) {  // pos += M_p
+  bool any_neg = false;
+  for (uint64_t x : neg.w) any_neg |= x != 0;
+  if (any_neg) neg.mod_mersenne(p);
+  if (any_neg && !pos.ge(neg)) {  // pos += M_p
     Big m(nw);
     for (uint32_t b = 0; b < p; ++b) m.w[b / 64] |= uint64_t(1) << (b % 64);
     Big t = pos;
@@ -501,22 +533,33 @@ std::vector<uint8_t> residue_from_state(uint32_t p, int ln, const int64_t* digit
     }
     pos = t;
   }
-  pos.sub(neg);
-  pos.mod_mersenne(p);
+  if (any_neg) {
+    pos.sub(neg);
+    pos.mod_mersenne(p);
+  }
   const size_t nbytes = (p + 7) / 8;
   std::vector<uint8_t> out(nbytes, 0);
-  for (size_t i = 0; i < nbytes; ++i) out[i] = (uint8_t)(pos.w[i / 8] >> (8 * (i % 8)));
+  for (size_t i = 0; i < nbytes; i += 8) {
+    const uint64_t x = pos.w[i / 8];
+    for (size_t k = 0; k < 8 && i + k < nbytes; ++k) out[i + k] = (uint8_t)(x >> (8 * k));
+  }
   return out;
 }
 
 void digits_from_residue(uint32_t p, int ln, const uint8_t* r, size_t len, int64_t* digits) {
-  auto bit = [&](uint64_t b) -> uint64_t { return (b / 8 < len) ? ((r[b / 8] >> (b % 8)) & 1u) : 0u; };
+  // digit j: the wd <= 47 bits from bit b0, read as one little-endian 64-bit
+  // window starting at byte b0 / 8 (bytes past len read as 0)
   const uint64_t n = uint64_t(1) << ln;
   for (uint64_t j = 0; j < n; ++j) {
     const uint64_t b0 = digit_bitpos(p, ln, j), wd = digit_bitpos(p, ln, j + 1) - b0;
-    uint64_t v = 0;
-    for (unsigned k = 0; k < wd; ++k) v |= bit(b0 + k) << k;
-    digits[j] = (int64_t)v;
+    const uint64_t at = b0 / 8;
+    uint64_t x = 0;
+    if (at + 8 <= len) {
+      memcpy(&x, r + at, 8);
+    } else {
+      for (uint64_t k = 0; at + k < len && k < 8; ++k) x |= (uint64_t)r[at + k] << (8 * k);
+    }
+    digits[j] = (int64_t)((x >> (b0 % 8)) & ((uint64_t(1) << wd) - 1));
   }
 }
 
