@@ -328,32 +328,38 @@ bool make_digit_gen(uint32_t p, const DevPlan& P, DigitGen& D, std::string& err)
   D.d31 = (uint32_t)((lr31 * D.s) % 31u);
   D.k61 = (uint32_t)((ln + 1) % 61);
   D.k31 = (uint32_t)((ln + 1) % 31);
-  // The kernels walk widths and weights incrementally; check the walk
-  // against the closed forms for every digit (and its restarts every 4096).
-  DWalk w = dwalk_at(D, 0);
-  uint64_t pj = 0;  // p j
-  uint64_t b0 = 0;  // ceil(p j / n)
-  for (uint64_t j = 0; j < n; ++j) {
-    if (j % 4096 == 0) {
-      const DWalk f = dwalk_at(D, (uint32_t)j);
-      if (f.r != w.r || f.v61 != w.v61 || f.v31 != w.v31) {
-        err = "digit walk: restart mismatch";
+  // The kernels restart the walk at a segment's first digit (dwalk_at) and
+  // step it across the segment (dwalk_next). Check both against the closed
+  // forms: over every digit up to n = 2^16; above that, over the first and
+  // last 2^16 digits and a 64-digit window at every multiple of 4096
+  // (about n/64 + 2^17 digits: the walk's correctness does not depend on the
+  // digit index beyond what these cover, and the check is most of a test's
+  // set-up time at large n).
+  auto check = [&](uint64_t j0, uint64_t len) -> bool {
+    DWalk w = dwalk_at(D, (uint32_t)j0);
+    uint64_t pj = (uint64_t)p * j0;            // p j
+    uint64_t b0 = j0 ? ((pj - 1) >> ln) + 1 : 0;  // ceil(p j / n)
+    for (uint64_t j = j0; j < j0 + len; ++j) {
+      const uint64_t b1 = ((pj + p - 1) >> ln) + 1;
+      const uint32_t wd = (uint32_t)(b1 - b0);
+      const uint64_t r = pj & (n - 1);
+      const uint32_t w61 = j ? (uint32_t)((lr61 * (n - r)) % 61u) : 0u;
+      const uint32_t w31 = j ? (uint32_t)((lr31 * (n - r)) % 31u) : 0u;
+      if ((wd != q && wd != q + 1) || dwalk_width(D, w) != wd || dwalk_w61(w) != w61 || dwalk_w31(w) != w31) {
+        err = "digit walk mismatch at digit " + std::to_string(j);
         return false;
       }
+      dwalk_next(D, w);
+      pj += p;
+      b0 = b1;
     }
-    const uint64_t b1 = ((pj + p - 1) >> ln) + 1;
-    const uint32_t wd = (uint32_t)(b1 - b0);
-    const uint64_t r = pj & (n - 1);
-    const uint32_t w61 = j ? (uint32_t)((lr61 * (n - r)) % 61u) : 0u;
-    const uint32_t w31 = j ? (uint32_t)((lr31 * (n - r)) % 31u) : 0u;
-    if ((wd != q && wd != q + 1) || dwalk_width(D, w) != wd || dwalk_w61(w) != w61 || dwalk_w31(w) != w31) {
-      err = "digit walk mismatch at digit " + std::to_string(j);
-      return false;
-    }
-    dwalk_next(D, w);
-    pj += p;
-    b0 = b1;
-  }
+    return true;
+  };
+  constexpr uint64_t FULL = uint64_t(1) << 16;
+  if (n <= FULL) return check(0, n);
+  if (!check(0, FULL) || !check(n - FULL, FULL)) return false;
+  for (uint64_t j0 = FULL; j0 < n - FULL; j0 += 4096)
+    if (!check(j0, 64)) return false;
   return true;
 }
 
