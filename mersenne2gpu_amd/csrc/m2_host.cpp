@@ -483,16 +483,19 @@ std::vector<uint8_t> residue_from_state(uint32_t p, int ln, const int64_t* digit
     if (neg.w.empty()) neg.w.assign(nw, 0);
     neg.add_at(bit, v);
   };
-  // Digits that are all non-negative and below 2^62 (every state the kernels
-  // leave) are summed in one streaming pass: a 128-bit accumulator holds the
-  // bits from word `wi` on, and each digit is added at its bit offset in it.
+  // Digits that are all non-negative and below 2^47 (every state the kernels
+  // leave: digit widths are at most 45 bits) are summed in one streaming
+  // pass: a 128-bit accumulator holds the bits from word `wi` on, and each
+  // digit is added at its bit offset (< 64) in it. Between two completed
+  // words at most 64 digits arrive, each below 2^110 at its offset, so the
+  // accumulator stays below 2^117.
   bool stream = true;
-  for (uint64_t j = 0; j < ndigits && stream; ++j) stream = digits[j] >= 0 && (digits[j] >> 62) == 0;
+  for (uint64_t j = 0; j < ndigits && stream; ++j) stream = digits[j] >= 0 && (digits[j] >> 47) == 0;
   if (stream) {
     // Invariant: digit j's offset in the accumulator, ceil(p j / n) - 64 wi,
-    // is below 64; a digit adds at most 47 bits, so after it at most one
-    // word is complete. Branch-free: the current word is always written
-    // and the accumulator moves on when it is complete.
+    // is below 64; the next digit starts at most 46 bits further on, so after
+    // each digit at most one word is complete. Branch-free: the current word
+    // is always written and the accumulator moves on when it is complete.
     unsigned __int128 acc = 0;
     uint64_t wi = 0, pj = 0;
     const uint64_t n1 = (uint64_t(1) << ln) - 1;

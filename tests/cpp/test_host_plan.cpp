@@ -104,6 +104,15 @@ int main() {
         CHECK(residue_from_state(p, P.ln, d2.data(), n, &c, &at, 1) ==
                   residue_from_state(p, P.ln, d4.data(), n, nullptr, nullptr, 0),
               "p=%u pending carry", p);
+        // a digit far above its width (summed term by term) equals the same
+        // amount as a pending carry at that digit (digits streamed)
+        std::vector<int64_t> d5 = d2;
+        d5[3] += (int64_t)1 << 50;
+        Carry128 c50{uint64_t(1) << 50, 0};
+        uint64_t at3 = 3;
+        CHECK(residue_from_state(p, P.ln, d5.data(), n, nullptr, nullptr, 0) ==
+                  residue_from_state(p, P.ln, d2.data(), n, &c50, &at3, 1),
+              "p=%u large digit", p);
       }
     }
   }
