@@ -1034,9 +1034,11 @@ __device__ __forceinline__ uint32_t xcd_strip(uint32_t b, uint32_t n) {
 constexpr int OUTER_TW = 1 << PASS_MAX_LOG_R;
 
 // CA > 0: the pass is known at compile time to be A = CA levels over W =
-// 2^CLOGW columns of stride S = 2^CLOGS, in radix-2, 8, 8 steps (n = 2^24: 7,
-// 4, 16) or radix-8, 8, 8 steps (n = 2^20: 9, 2, 10), so its step geometry and
-// index arithmetic are constants; CA = 0 reads it from the arguments.
+// 2^CLOGW columns of stride S = 2^CLOGS, in radix-4, 8 steps (n = 2^21: 5, 6,
+// 15), radix-8, 8 steps (n = 2^22: 6, 5, 15; n = 2^23: 6, 5, 16), radix-2, 8,
+// 8 steps (n = 2^24: 7, 4, 16) or radix-8, 8, 8 steps (n = 2^20: 9, 2, 10),
+// so its step geometry and index arithmetic are constants; CA = 0 reads it
+// from the arguments.
 template <int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests,
                                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31,
@@ -1050,7 +1052,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
   const M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
-  static_assert(CA == 0 || CA == 7 || CA == 9, "the constant forms are radix 2, 8, 8 and 8, 8, 8");
+  static_assert(CA == 0 || CA == 5 || CA == 6 || CA == 7 || CA == 9,
+                "the constant forms are radix 4, 8; 8, 8; 2, 8, 8 and 8, 8, 8");
   if (CA) logW = CLOGW, logS = CLOGS;
   const DigitGen D = td.dgen;
   const int R = 1 << (CA ? CA : ps.A), W = 1 << logW;
@@ -1080,7 +1083,21 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   // the last step (not level 0 when there are two or more) writes global
   // memory directly
   auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
-  if constexpr (CA == 7) {
+  if constexpr (CA == 5) {
+    // relative levels 0 (no twiddles) and 2; spans 3, 28 back to back
+    lds_step<true, 4, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+    const uint32_t off = ps.steps[1].twoff;
+    edge_step<MP_THREADS, true, 8, false>(L61, L31, CA, CLOGW, 1, 2, z61, z31, gidx, tw61, tw31,
+                                          [&](int, int blk) { return off + (uint32_t)blk * 7u; });
+    return;
+  } else if constexpr (CA == 6) {
+    // relative levels 0 (no twiddles) and 3; spans 7, 56 back to back
+    lds_step<true, 8, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+    const uint32_t off = ps.steps[1].twoff;
+    edge_step<MP_THREADS, true, 8, false>(L61, L31, CA, CLOGW, 1, 3, z61, z31, gidx, tw61, tw31,
+                                          [&](int, int blk) { return off + (uint32_t)blk * 7u; });
+    return;
+  } else if constexpr (CA == 7) {
     // relative levels 0 (no twiddles), 1 and 4; spans 1, 14, 112 back to back
     lds_step<true, 2, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
     lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 1, T61 + 1, T31 + 1);
@@ -1113,9 +1130,11 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
 }
 
 // CA > 0: the pass is known at compile time to be A = CA levels over W =
-// 2^CLOGW columns of stride S = 2^CLOGS, in radix-2, 8, 8 steps (n = 2^24: 7,
-// 4, 16) or radix-8, 8, 8 steps (n = 2^20: 9, 2, 10), so its step geometry and
-// index arithmetic are constants; CA = 0 reads it from the arguments.
+// 2^CLOGW columns of stride S = 2^CLOGS, in radix-4, 8 steps (n = 2^21: 5, 6,
+// 15), radix-8, 8 steps (n = 2^22: 6, 5, 15; n = 2^23: 6, 5, 16), radix-2, 8,
+// 8 steps (n = 2^24: 7, 4, 16) or radix-8, 8, 8 steps (n = 2^20: 9, 2, 10),
+// so its step geometry and index arithmetic are constants; CA = 0 reads it
+// from the arguments.
 template <int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __restrict__ tests,
                                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31,
@@ -1132,7 +1151,8 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
   const DigitGen D = td.dgen;
   const int mode = td.mode;
-  static_assert(CA == 0 || CA == 7 || CA == 9, "the constant forms are radix 2, 8, 8 and 8, 8, 8");
+  static_assert(CA == 0 || CA == 5 || CA == 6 || CA == 7 || CA == 9,
+                "the constant forms are radix 4, 8; 8, 8; 2, 8, 8 and 8, 8, 8");
   if (CA) logW = CLOGW, logS = CLOGS;
   const int R = 1 << (CA ? CA : ps.A), W = 1 << logW;
   const uint32_t cg = xcd_strip(blockIdx.x, (1u << logS) >> logW);
@@ -1144,7 +1164,13 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
              [](int i) { return PADX(i); });
   ft.store(T61, T31, ntw);
   __syncthreads();
-  if constexpr (CA == 7) {
+  if constexpr (CA == 5) {
+    lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 2, T61 + 3, T31 + 3);
+    lds_step<false, 4, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+  } else if constexpr (CA == 6) {
+    lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 3, T61 + 7, T31 + 7);
+    lds_step<false, 8, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+  } else if constexpr (CA == 7) {
     lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 4, T61 + 15, T31 + 15);
     lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 1, T61 + 1, T31 + 1);
     lds_step<false, 2, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
@@ -1328,14 +1354,15 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
 
 // Middle pass: levels [ell0, ell0 + A) with no carry work. Level-ell0 block
 // j (of R * S elements) is split into S/W column groups.
-// CA > 0: the pass is known at compile time to be A = CA levels in two
-// radix-8 steps over W = 2^CLOGW columns of stride S = 2^CLOGS (n = 2^24: 6,
-// 5, 10), so its step geometry and index arithmetic are constants; CA = 0
-// reads it from the arguments.
+// CA > 0: the pass is known at compile time to be A = CA levels in two steps,
+// radix 8, 8 over W = 2^CLOGW columns of stride S = 2^CLOGS (n = 2^23 and
+// 2^24: 6, 5, 10) or radix 4, 8 (n = 2^21 and 2^22: 5, 6, 10), so its step
+// geometry and index arithmetic are constants; CA = 0 reads it from the
+// arguments.
 template <bool FWD, int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                              const e31* __restrict__ tw31, Pass ps, int logS, int logW) {
-  static_assert(CA == 0 || CA == 6, "the constant form is two radix-8 steps");
+  static_assert(CA == 0 || CA == 5 || CA == 6, "the constant forms are two steps: radix 4, 8 or 8, 8");
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
   __shared__ e31 L31[padded(PASS_MAX_ELEMS)];
   __shared__ e61 T61[OUTER_TW];
@@ -1356,7 +1383,16 @@ __global__ void __launch_bounds__(MP_THREADS) k_mid(const TestDesc* __restrict__
   gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
   stage_pass_twiddles(T61, T31, ps, 1, j0, tw61, tw31);
   __syncthreads();
-  if constexpr (CA != 0) {
+  if constexpr (CA == 5) {
+    // relative levels 0 and 2; their twiddle spans (3 and 28) at offsets 0 and 3
+    if constexpr (FWD) {
+      lds_step<true, 4, true>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+      lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 2, T61 + 3, T31 + 3);
+    } else {
+      lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 2, T61 + 3, T31 + 3);
+      lds_step<false, 4, true>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+    }
+  } else if constexpr (CA == 6) {
     // relative levels 0 and 3; their twiddle spans (7 and 56) at offsets 0 and 7
     if constexpr (FWD) {
       lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
@@ -1561,11 +1597,19 @@ bool centre_const(const DevPlan& P, int direct_in) {
   return true;
 }
 
-// The A of the outer-pass geometry that k_outer_fwd/k_outer_inv<A, ...> have
-// compiled in (7: radix-2, 8, 8 steps, W = 16, S = 2^16; 9: radix-8 steps,
-// W = 4, S = 2^10) that P's outer pass has, or 0.
+// Which outer-pass geometry compiled into k_outer_fwd/k_outer_inv<A, logW,
+// logS> P's outer pass has (the kernels' own constant forms: 5: radix-4, 8
+// steps, W = 64, S = 2^15; 6: radix-8, 8 steps, W = 32, S = 2^15 or 2^16
+// (returned as 6 and 16); 7: radix-2, 8, 8 steps, W = 16, S = 2^16; 9:
+// radix-8 steps, W = 4, S = 2^10), or 0.
 int outer_const(const DevPlan& P) {
   const Pass& o = P.outer;
+  if (o.ell0 == 0 && o.nsteps == 2 && o.steps[0].ell == 0 && o.steps[1].radix == 8) {
+    if (o.A == 5 && o.steps[0].radix == 4 && o.steps[1].ell == 2 && P.logW == 6 && P.logS == 15) return 5;
+    if (o.A == 6 && o.steps[0].radix == 8 && o.steps[1].ell == 3 && P.logW == 5 && P.logS == 15) return 6;
+    if (o.A == 6 && o.steps[0].radix == 8 && o.steps[1].ell == 3 && P.logW == 5 && P.logS == 16) return 16;
+    return 0;
+  }
   if (o.ell0 != 0 || o.nsteps != 3 || o.steps[0].ell != 0 || o.steps[1].radix != 8 || o.steps[2].radix != 8)
     return 0;
   if (o.A == 7 && o.steps[0].radix == 2 && o.steps[1].ell == 1 && o.steps[2].ell == 4 && P.logW == 4 && P.logS == 16)
@@ -1583,31 +1627,45 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
   hipError_t e;
   const uint32_t ngroups_outer = (1u << P.logS) >> P.logW;
   const dim3 blk(MP_THREADS), cblk(NT);
-  // the outer passes of n = 2^24 (levels 0..6 in radix-2, 8, 8 steps, W = 16,
-  // S = 2^16) and n = 2^20 (levels 0..8 in radix-8 steps, W = 4, S = 2^10)
-  // have kernels with that geometry built in
+  // the outer passes of the default plans of n = 2^20 .. 2^24 have kernels
+  // with their geometry built in (outer_const)
   const int oc = outer_const(P);
-  auto outer_fwd = oc == 7 ? k_outer_fwd<7, 4, 16> : oc == 9 ? k_outer_fwd<9, 2, 10> : k_outer_fwd<>;
-  auto outer_inv = oc == 7 ? k_outer_inv<7, 4, 16> : oc == 9 ? k_outer_inv<9, 2, 10> : k_outer_inv<>;
+  auto outer_fwd = oc == 5    ? k_outer_fwd<5, 6, 15>
+                   : oc == 6  ? k_outer_fwd<6, 5, 15>
+                   : oc == 16 ? k_outer_fwd<6, 5, 16>
+                   : oc == 7  ? k_outer_fwd<7, 4, 16>
+                   : oc == 9  ? k_outer_fwd<9, 2, 10>
+                              : k_outer_fwd<>;
+  auto outer_inv = oc == 5    ? k_outer_inv<5, 6, 15>
+                   : oc == 6  ? k_outer_inv<6, 5, 15>
+                   : oc == 16 ? k_outer_inv<6, 5, 16>
+                   : oc == 7  ? k_outer_inv<7, 4, 16>
+                   : oc == 9  ? k_outer_inv<9, 2, 10>
+                              : k_outer_inv<>;
   if ((e = launch("k_outer_fwd", outer_fwd, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
                   P.logW)) != hipSuccess)
     return e;
   const uint32_t nmid = P.has_mid ? (1u << P.outer.A) * ((1u << P.mid_logS) >> P.mid_logW) : 0;
-  // the middle pass of n = 2^24 (6 levels in two radix-8 steps, W = 32,
-  // S = 1024) has a kernel with that geometry built in
-  const bool m6 = P.has_mid && P.mid.A == 6 && P.mid.nsteps == 2 && P.mid.steps[0].radix == 8 &&
-                  P.mid.steps[1].radix == 8 && P.mid.steps[0].ell == P.mid.ell0 &&
-                  P.mid.steps[1].ell == P.mid.ell0 + 3 && P.mid_logW == 5 && P.mid_logS == 10;
-  if (P.has_mid && (e = launch("k_mid_fwd", m6 ? k_mid<true, 6, 5, 10> : k_mid<true>, dim3(nmid, nt), blk, tests,
-                               P.tw61, P.tw31, P.mid, P.mid_logS, P.mid_logW)) != hipSuccess)
+  // the middle passes of n = 2^23 and 2^24 (6 levels in two radix-8 steps,
+  // W = 32, S = 1024) and of n = 2^21 and 2^22 (5 levels in radix-4, 8
+  // steps, W = 64, S = 1024) have kernels with that geometry built in
+  const Pass& mp = P.mid;
+  const bool mtwo = P.has_mid && mp.nsteps == 2 && mp.steps[0].ell == mp.ell0 && mp.steps[1].radix == 8 &&
+                    P.mid_logS == 10;
+  const bool m6 = mtwo && mp.A == 6 && mp.steps[0].radix == 8 && mp.steps[1].ell == mp.ell0 + 3 && P.mid_logW == 5;
+  const bool m5 = mtwo && mp.A == 5 && mp.steps[0].radix == 4 && mp.steps[1].ell == mp.ell0 + 2 && P.mid_logW == 6;
+  auto mid_fwd = m6 ? k_mid<true, 6, 5, 10> : m5 ? k_mid<true, 5, 6, 10> : k_mid<true>;
+  auto mid_inv = m6 ? k_mid<false, 6, 5, 10> : m5 ? k_mid<false, 5, 6, 10> : k_mid<false>;
+  if (P.has_mid && (e = launch("k_mid_fwd", mid_fwd, dim3(nmid, nt), blk, tests, P.tw61, P.tw31, P.mid, P.mid_logS,
+                               P.mid_logW)) != hipSuccess)
     return e;
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre * nt <= P.cus ? 1 : 0;
   if ((e = launch("k_centre", centre_const<NT>(P, direct_in) ? k_centre<NT, 10> : k_centre<NT, 0>, dim3(ncentre, nt),
                   cblk, tests, P.ctw61, P.ctw31, P.crec, P.centre, direct_in)) != hipSuccess)
     return e;
-  if (P.has_mid && (e = launch("k_mid_inv", m6 ? k_mid<false, 6, 5, 10> : k_mid<false>, dim3(nmid, nt), blk, tests,
-                               P.tw61, P.tw31, P.mid, P.mid_logS, P.mid_logW)) != hipSuccess)
+  if (P.has_mid && (e = launch("k_mid_inv", mid_inv, dim3(nmid, nt), blk, tests, P.tw61, P.tw31, P.mid, P.mid_logS,
+                               P.mid_logW)) != hipSuccess)
     return e;
   return launch("k_outer_inv", outer_inv, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
                 P.logW);
