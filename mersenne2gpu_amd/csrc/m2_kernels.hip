@@ -1034,11 +1034,10 @@ __device__ __forceinline__ uint32_t xcd_strip(uint32_t b, uint32_t n) {
 constexpr int OUTER_TW = 1 << PASS_MAX_LOG_R;
 
 // CA > 0: the pass is known at compile time to be A = CA levels over W =
-// 2^CLOGW columns of stride S = 2^CLOGS, in radix-4, 8 steps (n = 2^21: 5, 6,
-// 15), radix-8, 8 steps (n = 2^22: 6, 5, 15; n = 2^23: 6, 5, 16), radix-2, 8,
-// 8 steps (n = 2^24: 7, 4, 16) or radix-8, 8, 8 steps (n = 2^20: 9, 2, 10),
-// so its step geometry and index arithmetic are constants; CA = 0 reads it
-// from the arguments.
+// 2^CLOGW columns of stride S = 2^CLOGS (the shapes of OUTER_SHAPES), in
+// radix-4, 8 steps (A = 5), radix-8, 8 (6), radix-2, 8, 8 (7), radix-4, 8, 8
+// (8) or radix-8, 8, 8 (9), so its step geometry and index arithmetic are
+// constants; CA = 0 reads it from the arguments.
 template <int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __restrict__ tests,
                                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31,
@@ -1052,8 +1051,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   M2_GLOBAL e31* __restrict__ z31 = gptr(td.z31);
   const M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
-  static_assert(CA == 0 || CA == 5 || CA == 6 || CA == 7 || CA == 9,
-                "the constant forms are radix 4, 8; 8, 8; 2, 8, 8 and 8, 8, 8");
+  static_assert(CA == 0 || (CA >= 5 && CA <= 9), "the constant forms are radix 4, 8; 8, 8; 2, 8, 8; 4, 8, 8; 8, 8, 8");
   if (CA) logW = CLOGW, logS = CLOGS;
   const DigitGen D = td.dgen;
   const int R = 1 << (CA ? CA : ps.A), W = 1 << logW;
@@ -1105,6 +1103,14 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
     edge_step<MP_THREADS, true, 8, false>(L61, L31, CA, CLOGW, 1, 4, z61, z31, gidx, tw61, tw31,
                                           [&](int, int blk) { return off + (uint32_t)blk * 7u; });
     return;
+  } else if constexpr (CA == 8) {
+    // relative levels 0 (no twiddles), 2 and 5; spans 3, 28, 224 back to back
+    lds_step<true, 4, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+    lds_step<true, 8, true>(L61, L31, CA, CLOGW, 1, 2, T61 + 3, T31 + 3);
+    const uint32_t off = ps.steps[2].twoff;
+    edge_step<MP_THREADS, true, 8, false>(L61, L31, CA, CLOGW, 1, 5, z61, z31, gidx, tw61, tw31,
+                                          [&](int, int blk) { return off + (uint32_t)blk * 7u; });
+    return;
   } else if constexpr (CA == 9) {
     // relative levels 0 (no twiddles), 3 and 6; spans 7, 56, 448 back to back
     lds_step<true, 8, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
@@ -1130,11 +1136,10 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
 }
 
 // CA > 0: the pass is known at compile time to be A = CA levels over W =
-// 2^CLOGW columns of stride S = 2^CLOGS, in radix-4, 8 steps (n = 2^21: 5, 6,
-// 15), radix-8, 8 steps (n = 2^22: 6, 5, 15; n = 2^23: 6, 5, 16), radix-2, 8,
-// 8 steps (n = 2^24: 7, 4, 16) or radix-8, 8, 8 steps (n = 2^20: 9, 2, 10),
-// so its step geometry and index arithmetic are constants; CA = 0 reads it
-// from the arguments.
+// 2^CLOGW columns of stride S = 2^CLOGS (the shapes of OUTER_SHAPES), in
+// radix-4, 8 steps (A = 5), radix-8, 8 (6), radix-2, 8, 8 (7), radix-4, 8, 8
+// (8) or radix-8, 8, 8 (9), so its step geometry and index arithmetic are
+// constants; CA = 0 reads it from the arguments.
 template <int CA = 0, int CLOGW = 0, int CLOGS = 0>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __restrict__ tests,
                                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31,
@@ -1151,8 +1156,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
   M2_GLOBAL Carry128* __restrict__ G = gptr(td.G);
   const DigitGen D = td.dgen;
   const int mode = td.mode;
-  static_assert(CA == 0 || CA == 5 || CA == 6 || CA == 7 || CA == 9,
-                "the constant forms are radix 4, 8; 8, 8; 2, 8, 8 and 8, 8, 8");
+  static_assert(CA == 0 || (CA >= 5 && CA <= 9), "the constant forms are radix 4, 8; 8, 8; 2, 8, 8; 4, 8, 8; 8, 8, 8");
   if (CA) logW = CLOGW, logS = CLOGS;
   const int R = 1 << (CA ? CA : ps.A), W = 1 << logW;
   const uint32_t cg = xcd_strip(blockIdx.x, (1u << logS) >> logW);
@@ -1174,6 +1178,10 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_inv(const TestDesc* __rest
     lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 4, T61 + 15, T31 + 15);
     lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 1, T61 + 1, T31 + 1);
     lds_step<false, 2, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
+  } else if constexpr (CA == 8) {
+    lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 5, T61 + 31, T31 + 31);
+    lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 2, T61 + 3, T31 + 3);
+    lds_step<false, 4, true, false>(L61, L31, CA, CLOGW, 1, 0, T61, T31);
   } else if constexpr (CA == 9) {
     lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 6, T61 + 63, T31 + 63);
     lds_step<false, 8, true>(L61, L31, CA, CLOGW, 1, 3, T61 + 7, T31 + 7);
@@ -1597,26 +1605,45 @@ bool centre_const(const DevPlan& P, int direct_in) {
   return true;
 }
 
-// Which outer-pass geometry compiled into k_outer_fwd/k_outer_inv<A, logW,
-// logS> P's outer pass has (the kernels' own constant forms: 5: radix-4, 8
-// steps, W = 64, S = 2^15; 6: radix-8, 8 steps, W = 32, S = 2^15 or 2^16
-// (returned as 6 and 16); 7: radix-2, 8, 8 steps, W = 16, S = 2^16; 9:
-// radix-8 steps, W = 4, S = 2^10), or 0.
+// Outer-pass shapes compiled into k_outer_fwd/k_outer_inv<A, logW, logS>: the
+// default (radix-8) plans' outer passes of n = 2^16 .. 2^24. The steps are
+// those make_pass gives A levels: radix 4, 8 (A = 5), 8, 8 (6), 2, 8, 8 (7),
+// 4, 8, 8 (8), 8, 8, 8 (9), the first at level 0.
+struct OuterShape {
+  int A, logW, logS;
+};
+constexpr OuterShape OUTER_SHAPES[] = {
+    {5, 6, 10}, {6, 5, 10}, {7, 4, 10}, {8, 3, 10}, {9, 2, 10},  // n = 2^16 .. 2^20
+    {5, 6, 15}, {6, 5, 15}, {6, 5, 16}, {7, 4, 16},              // n = 2^21 .. 2^24
+};
+constexpr int N_OUTER_SHAPES = sizeof(OUTER_SHAPES) / sizeof(OUTER_SHAPES[0]);
+
+// Index + 1 of P's outer-pass shape in OUTER_SHAPES, or 0.
 int outer_const(const DevPlan& P) {
   const Pass& o = P.outer;
-  if (o.ell0 == 0 && o.nsteps == 2 && o.steps[0].ell == 0 && o.steps[1].radix == 8) {
-    if (o.A == 5 && o.steps[0].radix == 4 && o.steps[1].ell == 2 && P.logW == 6 && P.logS == 15) return 5;
-    if (o.A == 6 && o.steps[0].radix == 8 && o.steps[1].ell == 3 && P.logW == 5 && P.logS == 15) return 6;
-    if (o.A == 6 && o.steps[0].radix == 8 && o.steps[1].ell == 3 && P.logW == 5 && P.logS == 16) return 16;
+  static const int first_radix[10] = {0, 0, 0, 0, 0, 4, 8, 2, 4, 8};
+  if (o.ell0 != 0 || o.A < 5 || o.A > 9 || o.nsteps != (o.A + 2) / 3 || o.steps[0].ell != 0 ||
+      o.steps[0].radix != first_radix[o.A])
     return 0;
-  }
-  if (o.ell0 != 0 || o.nsteps != 3 || o.steps[0].ell != 0 || o.steps[1].radix != 8 || o.steps[2].radix != 8)
-    return 0;
-  if (o.A == 7 && o.steps[0].radix == 2 && o.steps[1].ell == 1 && o.steps[2].ell == 4 && P.logW == 4 && P.logS == 16)
-    return 7;
-  if (o.A == 9 && o.steps[0].radix == 8 && o.steps[1].ell == 3 && o.steps[2].ell == 6 && P.logW == 2 && P.logS == 10)
-    return 9;
+  for (int k = 1; k < o.nsteps; ++k)
+    if (o.steps[k].radix != 8 || o.steps[k].ell != o.steps[k - 1].ell + (k == 1 ? (o.A - 3 * (o.nsteps - 1)) : 3))
+      return 0;
+  for (int i = 0; i < N_OUTER_SHAPES; ++i)
+    if (OUTER_SHAPES[i].A == o.A && OUTER_SHAPES[i].logW == P.logW && OUTER_SHAPES[i].logS == P.logS) return i + 1;
   return 0;
+}
+
+// The kernel instances for shape index oc (outer_const) or the generic ones.
+template <bool FWD, int I = 0>
+auto outer_kernel(int oc) {
+  if constexpr (I == N_OUTER_SHAPES) {
+    (void)oc;
+    return FWD ? k_outer_fwd<> : k_outer_inv<>;
+  } else {
+    constexpr OuterShape sh = OUTER_SHAPES[I];
+    if (oc == I + 1) return FWD ? k_outer_fwd<sh.A, sh.logW, sh.logS> : k_outer_inv<sh.A, sh.logW, sh.logS>;
+    return outer_kernel<FWD, I + 1>(oc);
+  }
 }
 
 // Launches the kernels of one iteration of every test in `tests` through
@@ -1627,21 +1654,11 @@ hipError_t iteration_nt(const DevPlan& P, const TestDesc* tests, uint32_t nt, F&
   hipError_t e;
   const uint32_t ngroups_outer = (1u << P.logS) >> P.logW;
   const dim3 blk(MP_THREADS), cblk(NT);
-  // the outer passes of the default plans of n = 2^20 .. 2^24 have kernels
+  // the outer passes of the default plans of n = 2^16 .. 2^24 have kernels
   // with their geometry built in (outer_const)
   const int oc = outer_const(P);
-  auto outer_fwd = oc == 5    ? k_outer_fwd<5, 6, 15>
-                   : oc == 6  ? k_outer_fwd<6, 5, 15>
-                   : oc == 16 ? k_outer_fwd<6, 5, 16>
-                   : oc == 7  ? k_outer_fwd<7, 4, 16>
-                   : oc == 9  ? k_outer_fwd<9, 2, 10>
-                              : k_outer_fwd<>;
-  auto outer_inv = oc == 5    ? k_outer_inv<5, 6, 15>
-                   : oc == 6  ? k_outer_inv<6, 5, 15>
-                   : oc == 16 ? k_outer_inv<6, 5, 16>
-                   : oc == 7  ? k_outer_inv<7, 4, 16>
-                   : oc == 9  ? k_outer_inv<9, 2, 10>
-                              : k_outer_inv<>;
+  auto outer_fwd = outer_kernel<true>(oc);
+  auto outer_inv = outer_kernel<false>(oc);
   if ((e = launch("k_outer_fwd", outer_fwd, dim3(ngroups_outer, nt), blk, tests, P.tw61, P.tw31, P.outer, P.logS,
                   P.logW)) != hipSuccess)
     return e;
@@ -1684,7 +1701,9 @@ hipError_t run_fused_nt(const DevPlan& P, const TestDesc* tests, uint64_t k, F&&
   hipError_t e;
   // the outer passes of n = 2^20 (levels 0..8 in three radix-8 steps, W = 4,
   // S = 1024) have kernels with that geometry built in
-  const bool c20 = outer_const(P) == 9;
+  const int oc = outer_const(P);
+  const bool c20 = oc > 0 && OUTER_SHAPES[oc - 1].A == 9 && OUTER_SHAPES[oc - 1].logW == 2 &&
+                   OUTER_SHAPES[oc - 1].logS == 10;
   const uint32_t ns = outer_strips(P);
   const uint32_t ncentre = (P.h >> P.centre.A) / 2;
   const int direct_in = ncentre <= P.cus ? 1 : 0;
