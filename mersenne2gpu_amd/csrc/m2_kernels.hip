@@ -908,7 +908,51 @@ constexpr int SMALL_WAVE_THREADS = 128;
 constexpr int SMALL_LDS_LOG_H = 11;
 constexpr int SMALL_LDS_TW = 1024;
 
-template <int LOGH, int NTH>
+// The single-workgroup plan's default transform of 2^L points, its steps at
+// compile time: radix 4 up to L = 10 (a radix-2 step first when L is odd),
+// radix 8 above (a radix-2 or radix-4 step first for L mod 3 = 1 or 2), as
+// make_pass lays them out. Step K at relative level LAM has its twiddles at
+// offset OFF of the flat table ((r - 1) 2^LAM entries per step, back to back)
+// and is run when K0 <= K < K1; level 0 has no twiddle products. Forward runs
+// the steps in order, inverse in reverse.
+constexpr int small_maxr(int L) { return L <= 10 ? 4 : 8; }
+constexpr int small_first_radix(int L) {
+  return small_maxr(L) == 4 ? (L % 2 ? 2 : 4) : (L % 3 == 1 ? 2 : (L % 3 == 2 ? 4 : 8));
+}
+template <bool FWD, int L, bool WAVE, int K0, int K1, int K = 0, int LAM = 0, int OFF = 0>
+__device__ __forceinline__ void small_steps(e61* L61, e31* L31, const e61* T61, const e31* T31) {
+  constexpr int R = K == 0 ? small_first_radix(L) : small_maxr(L);
+  constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  constexpr bool MORE = LAM + LR < L;
+  if constexpr (!FWD && MORE)
+    small_steps<FWD, L, WAVE, K0, K1, K + 1, LAM + LR, OFF + (1 << LAM) * (R - 1)>(L61, L31, T61, T31);
+  if constexpr (K >= K0 && K < K1)
+    lds_step<FWD, R, true, (LAM != 0), WAVE>(L61, L31, L, 0, 1, LAM, T61 + OFF, T31 + OFF);
+  if constexpr (FWD && MORE)
+    small_steps<FWD, L, WAVE, K0, K1, K + 1, LAM + LR, OFF + (1 << LAM) * (R - 1)>(L61, L31, T61, T31);
+}
+// Steps of that plan, and how many of its leading steps have their tables
+// within the first `cap` entries.
+constexpr int small_nsteps(int L) {
+  return small_maxr(L) == 4 ? (L + 1) / 2 : (L + 2) / 3;
+}
+constexpr int small_lds_steps(int L, int cap) {
+  int k = 0, lam = 0, lo = 0;
+  while (lam < L) {
+    const int r = k == 0 ? small_first_radix(L) : small_maxr(L);
+    const int span = (1 << lam) * (r - 1);
+    if (lo + span > cap) break;
+    lo += span;
+    lam += r == 8 ? 3 : (r == 4 ? 2 : 1);
+    ++k;
+  }
+  return k;
+}
+
+// CL > 0: the transform is known at compile time to be 2^CL points in the
+// default plan (small_steps), so its step geometry is constant; CL = 0 reads
+// the steps from the pass.
+template <int LOGH, int NTH, int CL = 0>
 __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ tests, const e61* __restrict__ tw61,
                                                const e31* __restrict__ tw31, const e61* __restrict__ sq61,
                                                const e31* __restrict__ sq31, Pass ps) {
@@ -929,10 +973,11 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
   __shared__ e61 S61[SN];
   __shared__ e31 S31[SN];
   __shared__ Carry128 gs[NSEG];
+  static_assert(CL == 0 || CL <= LOGH, "a constant plan of at most 2^LOGH points");
   const TestDesc td = tests[blockIdx.x];
   M2_GLOBAL e61* __restrict__ z61 = gptr(td.z61);
   const DigitGen D = td.dgen;
-  const int logH = ps.A;
+  const int logH = CL ? CL : ps.A;
   const int h = 1 << logH;
   const int nseg = h / SEG_ELEMS;
   // steps [0, kl) take their twiddles from LDS, the rest (GTW) from global
@@ -971,17 +1016,23 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
     for (int i = 0; i < SPT; ++i)
       if (s0 + i < nseg) seg_weight(sd[i], L61, L31, (s0 + i) * SEG_ELEMS, M[i]);
     __syncthreads();
-    lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31, 0, kl);
+    // constant plan: steps [0, KL) from LDS, the rest (GTW) from global memory
+    constexpr int NSC = CL ? small_nsteps(CL) : 0, KL = GTW ? small_lds_steps(CL, TN) : NSC;
+    if constexpr (CL != 0) small_steps<true, CL, WAVE, 0, KL>(L61, L31, T61, T31);
+    else lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31, 0, kl);
     if constexpr (GTW) {
-      lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, G61, G31, kl);
+      if constexpr (CL != 0) small_steps<true, CL, WAVE, KL, NSC>(L61, L31, G61, G31);
+      else lds_pass<true, true, true, WAVE>(L61, L31, 0, 1, ps, G61, G31, kl);
       square_self<true, WAVE, SQK>(L61, L31, logH, 1, [&](int, int q) { return sq61[q]; },
                                    [&](int, int q) { return sq31[q]; });
-      lds_pass<false, true, true, WAVE>(L61, L31, 0, 1, ps, G61, G31, 0, ps.nsteps - kl);
+      if constexpr (CL != 0) small_steps<false, CL, WAVE, KL, NSC>(L61, L31, G61, G31);
+      else lds_pass<false, true, true, WAVE>(L61, L31, 0, 1, ps, G61, G31, 0, ps.nsteps - kl);
     } else {
       square_self<true, WAVE, SQK>(L61, L31, logH, 1, [&](int, int q) { return S61[q]; },
                                    [&](int, int q) { return S31[q]; });
     }
-    lds_pass<false, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31, ps.nsteps - kl);
+    if constexpr (CL != 0) small_steps<false, CL, WAVE, 0, KL>(L61, L31, T61, T31);
+    else lds_pass<false, true, true, WAVE>(L61, L31, 0, 1, ps, T61, T31, ps.nsteps - kl);
     if (WAVE) __syncthreads();  // the carry reads both fields
 #pragma unroll
     for (int i = 0; i < SPT; ++i) {
@@ -1566,19 +1617,48 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == M
 // ---------------------------------------------------------------------------
 // Host-side launchers, used by the engine.
 
-hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
+// P.L when P's single-workgroup transform is the default plan of its length
+// (small_steps), which has a compiled-in instance k_small<·, ·, L> for every
+// length of the plan (L = 2..12), else 0.
+int small_const(const DevPlan& P) {
+  const Pass& o = P.outer;
+  const int L = P.L;
+  if (L < 2 || L > SMALL_LOG_H || o.ell0 != 0 || o.A != L || o.nsteps != small_nsteps(L)) return 0;
+  int ell = 0;
+  for (int k = 0; k < o.nsteps; ++k) {
+    const int r = k == 0 ? small_first_radix(L) : small_maxr(L);
+    if (o.steps[k].radix != r || o.steps[k].ell != ell) return 0;
+    ell += r == 8 ? 3 : (r == 4 ? 2 : 1);
+  }
+  return L;
+}
+
+// k_small instance for P: wave-sized up to 2^8 points, 512 threads above;
+// the default plans with their steps compiled in
+template <int I = 2, class F>
+hipError_t small_dispatch(const DevPlan& P, F&& go, int cl = -1) {
   constexpr int WL = SMALL_WAVE_LOG_H, WT = SMALL_WAVE_THREADS, LL = SMALL_LDS_LOG_H, T = SMALL_THREADS;
   static_assert(SMALL_LOG_H <= LL + 1, "one kernel for the lengths past the all-LDS ones");
-  if (P.L <= WL)
-    hipLaunchKernelGGL((k_small<WL, WT>), dim3(ntests), dim3(WT), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31,
-                       P.outer);
-  else if (P.L <= LL)
-    hipLaunchKernelGGL((k_small<LL, T>), dim3(ntests), dim3(T), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31,
-                       P.outer);
-  else
-    hipLaunchKernelGGL((k_small<SMALL_LOG_H, T>), dim3(ntests), dim3(T), 0, st, tests, P.tw61, P.tw31, P.sq61,
-                       P.sq31, P.outer);
-  return hipGetLastError();
+  if (cl < 0) cl = small_const(P);
+  if constexpr (I <= SMALL_LOG_H) {
+    if (cl == I) {
+      if constexpr (I <= WL) return go(k_small<WL, WT, I>, WT);
+      else if constexpr (I <= LL) return go(k_small<LL, T, I>, T);
+      else return go(k_small<SMALL_LOG_H, T, I>, T);
+    }
+    return small_dispatch<I + 1>(P, go, cl);
+  } else {
+    if (P.L <= WL) return go(k_small<WL, WT>, WT);
+    if (P.L <= LL) return go(k_small<LL, T>, T);
+    return go(k_small<SMALL_LOG_H, T>, T);
+  }
+}
+
+hipError_t launch_small(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st) {
+  return small_dispatch(P, [&](auto kernel, int nth) {
+    hipLaunchKernelGGL(kernel, dim3(ntests), dim3(nth), 0, st, tests, P.tw61, P.tw31, P.sq61, P.sq31, P.outer);
+    return hipGetLastError();
+  });
 }
 
 // Polls of a neighbour's launch counter before the fused pass gives up
@@ -1785,17 +1865,11 @@ hipError_t launch_iteration_events(const DevPlan& P, const TestDesc* tests, uint
 
 hipError_t launch_small_events(const DevPlan& P, const TestDesc* tests, uint32_t ntests, hipStream_t st,
                                hipEvent_t start, hipEvent_t stop) {
-  constexpr int WL = SMALL_WAVE_LOG_H, WT = SMALL_WAVE_THREADS, LL = SMALL_LDS_LOG_H, T = SMALL_THREADS;
-  if (P.L <= WL)
-    hipExtLaunchKernelGGL((k_small<WL, WT>), dim3(ntests), dim3(WT), 0, st, start, stop, 0, tests, P.tw61, P.tw31,
-                          P.sq61, P.sq31, P.outer);
-  else if (P.L <= LL)
-    hipExtLaunchKernelGGL((k_small<LL, T>), dim3(ntests), dim3(T), 0, st, start, stop, 0, tests, P.tw61, P.tw31,
-                          P.sq61, P.sq31, P.outer);
-  else
-    hipExtLaunchKernelGGL((k_small<SMALL_LOG_H, T>), dim3(ntests), dim3(T), 0, st, start, stop, 0, tests, P.tw61,
-                          P.tw31, P.sq61, P.sq31, P.outer);
-  return hipGetLastError();
+  return small_dispatch(P, [&](auto kernel, int nth) {
+    hipExtLaunchKernelGGL(kernel, dim3(ntests), dim3(nth), 0, st, start, stop, 0, tests, P.tw61, P.tw31, P.sq61,
+                          P.sq31, P.outer);
+    return hipGetLastError();
+  });
 }
 
 // Issue-rate probe for v_mad_u64_u32, the 32x32->64 multiply-add every field
