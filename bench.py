@@ -18,6 +18,12 @@ Prints one JSON line. Extra fields:
                 product) per second against the device's v_mad_u64_u32 issue
                 rate measured in the same run, and the dominant kernel's VALU
                 busy fraction per SIMD from the committed counter summary;
+  config1       the n = 512 workload: whole PRP tests (the seeded exponent
+                in [10499, 20983] and 19937) to completion, one at a time,
+                final Res64 against the exact fixture, seconds per test;
+  config3       the n = 2^24 workload: Lucas-Lehmer, Res64 at 100 and 1000
+                against the exact fixture, then µs/iter and each kernel's
+                mean duration, the dominant one against HBM peak;
   batched       the metric's batched half: 512 PRP tests (n = 2^8..2^16,
                 seed 1) for 2000 iterations each in one m2_batch, tests/s
                 against the plain form (one m2_test after another) measured
@@ -199,6 +205,76 @@ def batched_leg(ctx, seed: int, cap: int, sample: int):
                             len(idx))}
 
 
+def whole_test_leg(ctx, seed: int):
+    """Config 1: plain PRP tests at n = 512 run to completion, the seeded
+    exponent from [10499, 20983] and p = 19937 (a Mersenne prime), each
+    created, run for its p squarings and read (Res64 and verdict) alone;
+    every final Res64 checked against the exact fixture."""
+    import mersenne2gpu_amd as m2
+    from mersenne2gpu_amd import workloads as W
+    fx = _fixture("c1.json") or {"cases": []}
+    want = {(c["p"], c["mode"]): c["checkpoints"][str(c["final"])]["res64"] for c in fx["cases"]}
+    ps = [W.draw_uniform(seed, *W.C1_RANGE), 19937]
+    cases = []
+    for p in ps:
+        with m2.Test(ctx, p, "prp") as w:  # warm: first launch of this length's kernels
+            w.run(16)
+            w.sync()
+        t0 = time.perf_counter()
+        with m2.Test(ctx, p, "prp") as t:
+            t.run(p)
+            r64 = t.res64()
+            prime = t.is_prime()
+        dt = time.perf_counter() - t0
+        cases.append({"p": p, "iterations": p, "s": round(dt, 4), "us_per_iter": round(dt / p * 1e6, 3),
+                      "res64": r64, "match": r64 == want.get((p, "prp")),
+                      "verdict": "probable prime" if prime else "composite"})
+    log("config 1: %s" % json.dumps(cases))
+    return {"workload": "plain PRP tests to completion at n=512 (p from seed %d in [10499, 20983], and 19937); "
+                        "wall time of create + p squarings + Res64/verdict read, one test at a time" % seed,
+            "cases": cases, "ok": all(c["match"] for c in cases)}
+
+
+def large_test_leg(ctx, seed: int, steps: int):
+    """Config 3: the Lucas-Lehmer test at n = 2^24 (p from seed in
+    [285212677, 553648103]); Res64 at iterations 100 and 1000 against the
+    exact fixture, then `steps` more iterations timed with hipEvents and each
+    kernel's mean duration, the dominant one priced against HBM peak."""
+    import mersenne2gpu_amd as m2
+    from mersenne2gpu_amd import workloads as W
+    p = W.draw_uniform(seed, *W.C3_RANGE)
+    ln = W.transformsize(p)
+    h = 1 << (ln - 1)
+    fx = _fixture("c3.json") or {"cases": []}
+    want = {}
+    for c in fx["cases"]:
+        if c["p"] == p and c["mode"] == "ll":
+            want = {int(k): v["res64"] for k, v in c["checkpoints"].items()}
+    parity = {}
+    with m2.Test(ctx, p, "ll") as t:
+        done = 0
+        for k in (100, 1000):
+            t.run(k - done)
+            done = k
+            got = t.res64()
+            parity[str(k)] = {"res64": got, "match": got == want.get(k)}
+        ms = t.time(steps)
+        kt = {k: v * 1000.0 for k, v in t.kernel_times(10).items()}
+    us = ms * 1000.0 / steps
+    dom = max(kt, key=kt.get)
+    gbps = algorithmic_bytes(dom, h) / (kt[dom] * 1e-6) / 1e9
+    out = {"workload": "plain single LL test, p=%d (seed %d), n=2^%d, %d iterations timed after 1000 checked" % (
+               p, seed, ln, steps),
+           "us_per_iter": round(us, 3), "hbm_gbps_algorithmic": round(48.0 * 2 * h / (us * 1e-6) / 1e9, 1),
+           "per_kernel_us": {k: round(v, 3) for k, v in kt.items()},
+           "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS,
+                        "unit": "GB/s", "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                        "bytes_per_launch": algorithmic_bytes(dom, h), "us_per_launch": round(kt[dom], 3)},
+           "parity": parity, "ok": all(v["match"] for v in parity.values())}
+    log("config 3: %s" % json.dumps(out))
+    return out
+
+
 def _pct(xs, q):
     xs = sorted(xs)
     k = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
@@ -342,6 +418,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-served", action="store_true", help="skip the served leg")
+    ap.add_argument("--no-config1", action="store_true", help="skip the n = 512 whole-test leg")
+    ap.add_argument("--no-config3", action="store_true", help="skip the n = 2^24 Lucas-Lehmer leg")
+    ap.add_argument("--config3-steps", type=int, default=1000, help="timed iterations of the n = 2^24 leg")
     ap.add_argument("--batch-cap", type=int, default=2000, help="iterations per batched test")
     ap.add_argument("--batch-plain-sample", type=int, default=64, help="tests in the measured plain sample")
     ap.add_argument("--serve-count", type=int, default=400, help="requests of the serving stream")
@@ -439,6 +518,9 @@ def main() -> int:
             tk.run(k)
             return tk.res64()
 
+    legs = args.workload == "prp-n2p20"
+    config1 = None if (args.no_config1 or not legs) else whole_test_leg(ctx, args.seed)
+    config3 = None if (args.no_config3 or not legs) else large_test_leg(ctx, args.seed, args.config3_steps)
     batched = None if args.no_batched else batched_leg(ctx, args.seed, args.batch_cap, args.batch_plain_sample)
     served = None if args.no_served else served_leg(ctx, args.seed, args.serve_count, args.serve_iters, 200.0, 1000)
     cpu = None
@@ -484,12 +566,17 @@ def main() -> int:
                          "achieved_gmodmul_s": round(modmuls / (us_per_iter * 1e-6) / 1e9, 1),
                          "kernel": dom, "valu_busy": load_valu(dom, wl["traffic_key"]),
                          "valu_busy_source": _profile_source("valu.json")},
+        "config1": config1,
+        "config3": config3,
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
     ok = parity_ok
+    for leg in (config1, config3):
+        if leg:
+            ok = ok and leg["ok"]
     if batched:
         ok = ok and batched["res64_mismatched_fixture"] == 0 and batched["plain_res64_mismatched"] == 0
     if served:
