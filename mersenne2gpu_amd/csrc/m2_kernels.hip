@@ -139,6 +139,40 @@ __device__ __forceinline__ Carry128 gld(const M2_GLOBAL Carry128* p) { return Ca
 __device__ __forceinline__ void gst(M2_GLOBAL e61* p, e61 v) { p->a = v.a; p->b = v.b; }
 __device__ __forceinline__ void gst(M2_GLOBAL e31* p, e31 v) { p->a = v.a; p->b = v.b; }
 __device__ __forceinline__ void gst(M2_GLOBAL Carry128* p, Carry128 v) { p->lo = v.lo; p->hi = v.hi; }
+// Streaming forms (nontemporal loads and stores) for the spectrum of a kernel
+// that reads and writes it once per launch. Measured per kernel shape: they
+// pay where one workgroup per CU runs the whole grid (the n = 2^20 single
+// test's fused outer pass and 1024-thread centre) and cost where two
+// workgroups per CU overlap (batches, n >= 2^21), so only those kernels use
+// them.
+template <bool NTM>
+__device__ __forceinline__ e61 gld_s(const M2_GLOBAL e61* p) {
+  if constexpr (NTM) return e61{__builtin_nontemporal_load(&p->a), __builtin_nontemporal_load(&p->b)};
+  else return gld(p);
+}
+template <bool NTM>
+__device__ __forceinline__ e31 gld_s(const M2_GLOBAL e31* p) {
+  if constexpr (NTM) return e31{__builtin_nontemporal_load(&p->a), __builtin_nontemporal_load(&p->b)};
+  else return gld(p);
+}
+template <bool NTM>
+__device__ __forceinline__ void gst_s(M2_GLOBAL e61* p, e61 v) {
+  if constexpr (NTM) {
+    __builtin_nontemporal_store(v.a, &p->a);
+    __builtin_nontemporal_store(v.b, &p->b);
+  } else {
+    gst(p, v);
+  }
+}
+template <bool NTM>
+__device__ __forceinline__ void gst_s(M2_GLOBAL e31* p, e31 v) {
+  if constexpr (NTM) {
+    __builtin_nontemporal_store(v.a, &p->a);
+    __builtin_nontemporal_store(v.b, &p->b);
+  } else {
+    gst(p, v);
+  }
+}
 
 template <class CP>
 __device__ __forceinline__ i128 load_carry(CP G, uint32_t i) {
@@ -573,8 +607,9 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, S61 
 // the thread's own elements at global index gidx(LDS index) and its twiddles
 // read from the global tables at twi(b, blk) + e - 1. A first step so needs
 // no barrier before it (each wave starts on its own loads as they arrive),
-// and a last step no LDS write-back and store loop after it.
-template <int NT, bool FWD, int RADIX, bool LOAD, class GI, class TI>
+// and a last step no LDS write-back and store loop after it. NTM: the
+// spectrum moves with streaming loads and stores (gld_s, gst_s).
+template <int NT, bool FWD, int RADIX, bool LOAD, bool NTM = false, class GI, class TI>
 __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW, int nb, int lam,
                                           M2_GLOBAL e61* z61, M2_GLOBAL e31* z31, GI gidx,
                                           const e61* __restrict__ tw61, const e31* __restrict__ tw31, TI twi) {
@@ -607,7 +642,7 @@ __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW
       for (int k = 0; k < KMAX; ++k)
         if (ln.tid + k * ln.n < total) {
 #pragma unroll
-          for (int e = 0; e < RADIX; ++e) v[k][e] = gld(z61 + gidx(base[k] + e * stride));
+          for (int e = 0; e < RADIX; ++e) v[k][e] = gld_s<NTM>(z61 + gidx(base[k] + e * stride));
         }
     }
 #pragma unroll
@@ -634,7 +669,7 @@ __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW
 #pragma unroll
       for (int e = 0; e < RADIX; ++e) {
         if (LOAD) L61[PADX(base[k] + e * stride)] = v[k][e];
-        else gst(z61 + gidx(base[k] + e * stride), v[k][e]);
+        else gst_s<NTM>(z61 + gidx(base[k] + e * stride), v[k][e]);
       }
     }
   } else {
@@ -644,7 +679,7 @@ __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW
       for (int k = 0; k < KMAX; ++k)
         if (ln.tid + k * ln.n < total) {
 #pragma unroll
-          for (int e = 0; e < RADIX; ++e) x[k][e] = gld(z31 + gidx(base[k] + e * stride));
+          for (int e = 0; e < RADIX; ++e) x[k][e] = gld_s<NTM>(z31 + gidx(base[k] + e * stride));
         }
     }
 #pragma unroll
@@ -671,7 +706,7 @@ __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW
 #pragma unroll
       for (int e = 0; e < RADIX; ++e) {
         if (LOAD) L31[PADX(base[k] + e * stride)] = x[k][e];
-        else gst(z31 + gidx(base[k] + e * stride), x[k][e]);
+        else gst_s<NTM>(z31 + gidx(base[k] + e * stride), x[k][e]);
       }
     }
   }
@@ -1308,8 +1343,9 @@ __device__ __forceinline__ i128 carry_take(M2_GLOBAL uint64_t* cf, uint32_t i, u
 
 // CA > 0: the pass is known at compile time to be A = CA levels in radix-8
 // steps over W = 2^CLOGW columns of stride S = 2^CLOGS (the n = 2^20 single
-// test: 9, 2, 10), so its step geometry and index arithmetic are constants;
-// CA = 0 reads it from the arguments.
+// test: 9, 2, 10), so its step geometry and index arithmetic are constants,
+// and its spectrum moves with streaming loads and stores (gld_s, gst_s);
+// CA = 0 reads the geometry from the arguments.
 template <int CA, int CLOGW, int CLOGS>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __restrict__ tests,
                                                      const e61* __restrict__ tw61, const e31* __restrict__ tw31,
@@ -1346,7 +1382,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   auto twl = [&](int, int blk) { return sl.twoff + (uint32_t)blk * (uint32_t)(sl.radix - 1); };
   FlatTwiddles ft;
   const int ntw = outer_twiddles(ft, ps, tw61, tw31);
-  if (CA) edge_step<MP_THREADS, false, 8, true>(L61, L31, 9, CLOGW, 1, 6, z61, z31, gidx, tw61, tw31, twl);
+  if (CA) edge_step<MP_THREADS, false, 8, true, true>(L61, L31, 9, CLOGW, 1, 6, z61, z31, gidx, tw61, tw31, twl);
   else if (edges)
     edge_pass_step<MP_THREADS, false, true>(L61, L31, logW, 1, ps, ns - 1, z61, z31, gidx, tw61, tw31, twl);
   else gather2<4>(L61, L31, z61, z31, R * W, gidx, [](int i) { return PADX(i); });
@@ -1397,7 +1433,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   if (CA) {
     lds_step<true, 8, true, false>(L61, L31, 9, CLOGW, 1, 0, T61, T31);
     lds_step<true, 8, true>(L61, L31, 9, CLOGW, 1, 3, T61 + 7, T31 + 7);
-    edge_step<MP_THREADS, true, 8, false>(L61, L31, 9, CLOGW, 1, 6, z61, z31, gidx, tw61, tw31, twl);
+    edge_step<MP_THREADS, true, 8, false, true>(L61, L31, 9, CLOGW, 1, 6, z61, z31, gidx, tw61, tw31, twl);
     return;
   }
   lds_pass<true, true, true>(L61, L31, logW, 1, ps, T61, T31, 0, edges ? ns - 1 : ns);
@@ -1492,6 +1528,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == M
   static_assert(CC == 0 || (CC == 10 && (NT == MP_THREADS_WIDE || NT == MP_THREADS_NARROW)),
                 "the constant forms are the 2^10 centres");
   constexpr bool WIDE = NT == MP_THREADS_WIDE;
+  // the constant 1024-thread form (the n = 2^20 single test, one workgroup
+  // per CU) streams its spectrum and its register twiddles (gld_s, gst_s)
+  constexpr bool STREAM = WIDE && CC != 0;
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
   __shared__ e61 T61[CENTRE_EARLY_TW];
@@ -1547,12 +1586,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == M
     if (ln.f61) {
       const M2_GLOBAL e61* src = gptr(r61 + nE) + ln.tid;
 #pragma unroll
-      for (int k = 0; k < NSLOT; ++k) tw[k] = gld(src + k * CS.role_threads);
+      for (int k = 0; k < NSLOT; ++k) tw[k] = gld_s<STREAM>(src + k * CS.role_threads);
     } else {
       const M2_GLOBAL e31* src = gptr(r31 + nE) + ln.tid;
 #pragma unroll
       for (int k = 0; k < NSLOT; ++k) {
-        const e31 w = gld(src + k * CS.role_threads);
+        const e31 w = gld_s<STREAM>(src + k * CS.role_threads);
         tw[k] = e61{(uint64_t)w.a | ((uint64_t)w.b << 32), 0};
       }
     }
@@ -1560,7 +1599,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == M
   if constexpr (NT == MP_THREADS_WIDE) load_reg_twiddles();
   FlatTwiddles ft;
   ft.load(r61, r31, nE);
-  if constexpr (CC != 0 && WIDE) edge_step<NT, true, 4, true>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
+  if constexpr (CC != 0 && WIDE) edge_step<NT, true, 4, true, STREAM>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
   else if (!CC && edge_in) edge_pass_step<NT, true, true>(L61, L31, 0, nb, ps, 0, z61, z31, gidx, r61, r31, tw0);
   else gather2<PASS_MAX_ELEMS / NT>(L61, L31, z61, z31, nb << logC, gidx, [](int i) { return PADX(i); });
   ft.store(T61, T31, nE);
@@ -1592,7 +1631,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == M
     lds_step<false, 4, true>(L61, L31, CC, 0, 2, 6, T61 + 126, T31 + 126);
     lds_step<false, 4, true>(L61, L31, CC, 0, 2, 4, T61 + 30, T31 + 30);
     lds_step<false, 4, true>(L61, L31, CC, 0, 2, 2, T61 + 6, T31 + 6);
-    edge_step<NT, false, 4, false>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
+    edge_step<NT, false, 4, false, STREAM>(L61, L31, CC, 0, 2, 0, z61, z31, gidx, r61, r31, tw0);
     return;
   }
   if constexpr (CC != 0 && !WIDE) {
