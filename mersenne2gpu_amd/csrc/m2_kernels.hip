@@ -1372,6 +1372,9 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
   // this launch's count: every thread reads the strip's counter before thread
   // 0 advances it (after the first barrier), and uses it only after the
   // inverse pass, so the load's latency is hidden behind the data loads
+  // the M61 waves (first half) carry the longer dependency chains: they get
+  // the SIMD's issue slots first
+  if (threadIdx.x < blockDim.x / 2) __builtin_amdgcn_s_setprio(2);
   const uint32_t epoch = __hip_atomic_load(sync + cg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
   auto gidx = [&](int i) { return ((uint32_t)(i >> logW) << logS) + col0 + (uint32_t)(i & (W - 1)); };
   // the inverse pass's first step (the forward's last) reads global memory
@@ -1581,6 +1584,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == M
   constexpr CentreSlots CS = centre_slots(NT);
   constexpr int NSLOT = centre_tw_slots<NT>();
   const Lanes<true> ln;
+  // the M61 waves (first half) carry the longer dependency chains: they get
+  // the SIMD's issue slots first
+  if (threadIdx.x < blockDim.x / 2) __builtin_amdgcn_s_setprio(2);
   e61 tw[NSLOT];
   auto load_reg_twiddles = [&]() {
     if (ln.f61) {
