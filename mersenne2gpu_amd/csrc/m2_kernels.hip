@@ -132,19 +132,27 @@ __device__ __forceinline__ M2_GLOBAL T* gptr(T* p) {
 }
 
 // Element copies between global memory and registers/LDS (structure
-// assignment does not cross address spaces).
+// assignment does not cross address spaces). Stores of spectra and digits are
+// streaming (nontemporal): their reader is the next launch, on any XCD, and
+// so they need not stay in this XCD's L2 (every shape measured 1.5-2% faster
+// per iteration at n = 2^20..2^24, batched or not).
 __device__ __forceinline__ e61 gld(const M2_GLOBAL e61* p) { return e61{p->a, p->b}; }
 __device__ __forceinline__ e31 gld(const M2_GLOBAL e31* p) { return e31{p->a, p->b}; }
 __device__ __forceinline__ Carry128 gld(const M2_GLOBAL Carry128* p) { return Carry128{p->lo, p->hi}; }
-__device__ __forceinline__ void gst(M2_GLOBAL e61* p, e61 v) { p->a = v.a; p->b = v.b; }
-__device__ __forceinline__ void gst(M2_GLOBAL e31* p, e31 v) { p->a = v.a; p->b = v.b; }
+__device__ __forceinline__ void gst(M2_GLOBAL e61* p, e61 v) {
+  __builtin_nontemporal_store(v.a, &p->a);
+  __builtin_nontemporal_store(v.b, &p->b);
+}
+__device__ __forceinline__ void gst(M2_GLOBAL e31* p, e31 v) {
+  __builtin_nontemporal_store(v.a, &p->a);
+  __builtin_nontemporal_store(v.b, &p->b);
+}
 __device__ __forceinline__ void gst(M2_GLOBAL Carry128* p, Carry128 v) { p->lo = v.lo; p->hi = v.hi; }
-// Streaming forms (nontemporal loads and stores) for the spectrum of a kernel
-// that reads and writes it once per launch. Measured per kernel shape: they
-// pay where one workgroup per CU runs the whole grid (the n = 2^20 single
-// test's fused outer pass and 1024-thread centre) and cost where two
-// workgroups per CU overlap (batches, n >= 2^21), so only those kernels use
-// them.
+// Streaming loads of a kernel's spectrum (NTM). Measured per kernel shape:
+// they pay where one workgroup per CU runs the whole grid (the n = 2^20
+// single test's fused outer pass and 1024-thread centre: 49.2 -> 45.4
+// us/iteration) and cost 1-3% where two workgroups per CU overlap (batches,
+// n >= 2^21), so only those kernels use them.
 template <bool NTM>
 __device__ __forceinline__ e61 gld_s(const M2_GLOBAL e61* p) {
   if constexpr (NTM) return e61{__builtin_nontemporal_load(&p->a), __builtin_nontemporal_load(&p->b)};
@@ -154,24 +162,6 @@ template <bool NTM>
 __device__ __forceinline__ e31 gld_s(const M2_GLOBAL e31* p) {
   if constexpr (NTM) return e31{__builtin_nontemporal_load(&p->a), __builtin_nontemporal_load(&p->b)};
   else return gld(p);
-}
-template <bool NTM>
-__device__ __forceinline__ void gst_s(M2_GLOBAL e61* p, e61 v) {
-  if constexpr (NTM) {
-    __builtin_nontemporal_store(v.a, &p->a);
-    __builtin_nontemporal_store(v.b, &p->b);
-  } else {
-    gst(p, v);
-  }
-}
-template <bool NTM>
-__device__ __forceinline__ void gst_s(M2_GLOBAL e31* p, e31 v) {
-  if constexpr (NTM) {
-    __builtin_nontemporal_store(v.a, &p->a);
-    __builtin_nontemporal_store(v.b, &p->b);
-  } else {
-    gst(p, v);
-  }
 }
 
 template <class CP>
@@ -608,7 +598,7 @@ __device__ __forceinline__ void square_cross(e61* L61, e31* L31, int logCb, S61 
 // read from the global tables at twi(b, blk) + e - 1. A first step so needs
 // no barrier before it (each wave starts on its own loads as they arrive),
 // and a last step no LDS write-back and store loop after it. NTM: the
-// spectrum moves with streaming loads and stores (gld_s, gst_s).
+// spectrum is read with streaming loads (gld_s).
 template <int NT, bool FWD, int RADIX, bool LOAD, bool NTM = false, class GI, class TI>
 __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW, int nb, int lam,
                                           M2_GLOBAL e61* z61, M2_GLOBAL e31* z31, GI gidx,
@@ -669,7 +659,7 @@ __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW
 #pragma unroll
       for (int e = 0; e < RADIX; ++e) {
         if (LOAD) L61[PADX(base[k] + e * stride)] = v[k][e];
-        else gst_s<NTM>(z61 + gidx(base[k] + e * stride), v[k][e]);
+        else gst(z61 + gidx(base[k] + e * stride), v[k][e]);
       }
     }
   } else {
@@ -706,7 +696,7 @@ __device__ __forceinline__ void edge_step(e61* L61, e31* L31, int logR, int logW
 #pragma unroll
       for (int e = 0; e < RADIX; ++e) {
         if (LOAD) L31[PADX(base[k] + e * stride)] = x[k][e];
-        else gst_s<NTM>(z31 + gidx(base[k] + e * stride), x[k][e]);
+        else gst(z31 + gidx(base[k] + e * stride), x[k][e]);
       }
     }
   }
@@ -1344,7 +1334,7 @@ __device__ __forceinline__ i128 carry_take(M2_GLOBAL uint64_t* cf, uint32_t i, u
 // CA > 0: the pass is known at compile time to be A = CA levels in radix-8
 // steps over W = 2^CLOGW columns of stride S = 2^CLOGS (the n = 2^20 single
 // test: 9, 2, 10), so its step geometry and index arithmetic are constants,
-// and its spectrum moves with streaming loads and stores (gld_s, gst_s);
+// and its spectrum is read with streaming loads (gld_s);
 // CA = 0 reads the geometry from the arguments.
 template <int CA, int CLOGW, int CLOGS>
 __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __restrict__ tests,
@@ -1532,7 +1522,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == M
                 "the constant forms are the 2^10 centres");
   constexpr bool WIDE = NT == MP_THREADS_WIDE;
   // the constant 1024-thread form (the n = 2^20 single test, one workgroup
-  // per CU) streams its spectrum and its register twiddles (gld_s, gst_s)
+  // per CU) reads its spectrum and its register twiddles with streaming
+  // loads (gld_s)
   constexpr bool STREAM = WIDE && CC != 0;
   __shared__ e61 L61[padded(2 << CENTRE_LOG_C)];
   __shared__ e31 L31[padded(2 << CENTRE_LOG_C)];
