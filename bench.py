@@ -518,11 +518,14 @@ def main() -> int:
             tk.run(k)
             return tk.res64()
 
+    # the other workloads and legs belong to the headline (n = 2^20) line
     legs = args.workload == "prp-n2p20"
     config1 = None if (args.no_config1 or not legs) else whole_test_leg(ctx, args.seed)
     config3 = None if (args.no_config3 or not legs) else large_test_leg(ctx, args.seed, args.config3_steps)
-    batched = None if args.no_batched else batched_leg(ctx, args.seed, args.batch_cap, args.batch_plain_sample)
-    served = None if args.no_served else served_leg(ctx, args.seed, args.serve_count, args.serve_iters, 200.0, 1000)
+    batched = None if (args.no_batched or not legs) else batched_leg(ctx, args.seed, args.batch_cap,
+                                                                        args.batch_plain_sample)
+    served = None if (args.no_served or not legs) else served_leg(ctx, args.seed, args.serve_count, args.serve_iters,
+                                                                   200.0, 1000)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, mode, cpu_iters[0], cpu_iters[1], gpu_res64)
