@@ -49,10 +49,10 @@ PLANS = {
 
 # SQ_INSTS_VALU per wave, rocprofv3 (profiles named above)
 ISSUED = {
-    "k_centre<1024, 10>": 2683, "k_outer_fused<9, 2, 10>": 3842, "k_outer_fwd<9, 2, 10>": 1938,
+    "k_centre<1024, 10>": 2684, "k_outer_fused<9, 2, 10>": 3843, "k_outer_fwd<9, 2, 10>": 1937,
     "k_outer_inv<9, 2, 10>": 2325,
-    "k_centre<512, 10>": 4983, "k_outer_inv<7, 4, 16>": 2213, "k_outer_fwd<7, 4, 16>": 1705,
-    "k_mid<true, 6, 5, 10>": 1177, "k_mid<false, 6, 5, 10>": 1194,
+    "k_centre<512, 10>": 4983, "k_outer_inv<7, 4, 16>": 2347, "k_outer_fwd<7, 4, 16>": 1863,
+    "k_mid<true, 6, 5, 10>": 1176, "k_mid<false, 6, 5, 10>": 1193,
 }
 
 
