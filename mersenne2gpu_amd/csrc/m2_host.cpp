@@ -335,8 +335,13 @@ bool make_digit_gen(uint32_t p, const DevPlan& P, DigitGen& D, std::string& err)
   // (about n/64 + 2^17 digits: the walk's correctness does not depend on the
   // digit index beyond what these cover, and the check is most of a test's
   // set-up time at large n).
+  // The carrying kernels also take a segment's metadata (seg_meta: shifts,
+  // unweighting rotations and width bits of its 8 digits, each count possibly
+  // holding its modulus for 0) from its first digit; it is checked at every
+  // segment start inside the same windows (all multiples of 8).
   auto check = [&](uint64_t j0, uint64_t len) -> bool {
     DWalk w = dwalk_at(D, (uint32_t)j0);
+    SegMeta M{};
     uint64_t pj = (uint64_t)p * j0;            // p j
     uint64_t b0 = j0 ? ((pj - 1) >> ln) + 1 : 0;  // ceil(p j / n)
     for (uint64_t j = j0; j < j0 + len; ++j) {
@@ -347,6 +352,15 @@ bool make_digit_gen(uint32_t p, const DevPlan& P, DigitGen& D, std::string& err)
       const uint32_t w31 = j ? (uint32_t)((lr31 * (n - r)) % 31u) : 0u;
       if ((wd != q && wd != q + 1) || dwalk_width(D, w) != wd || dwalk_w61(w) != w61 || dwalk_w31(w) != w31) {
         err = "digit walk mismatch at digit " + std::to_string(j);
+        return false;
+      }
+      if ((j & (2 * SEG_ELEMS - 1)) == 0) M = seg_meta(D, (uint32_t)j);
+      const uint32_t m = M.m[j & (2 * SEG_ELEMS - 1)];
+      const uint32_t u61 = meta_u61(m), u31 = meta_u31(m);  // left rotations by -(w + ln + 1)
+      if (meta_w61(m) > 61u || meta_w31(m) > 31u || u61 > 61u || u31 > 31u || meta_w61(m) % 61u != w61 ||
+          meta_w31(m) % 31u != w31 || (u61 + w61 + D.k61) % 61u != 0 || (u31 + w31 + D.k31) % 31u != 0 ||
+          meta_wide(m) != wd - q) {
+        err = "segment metadata mismatch at digit " + std::to_string(j);
         return false;
       }
       dwalk_next(D, w);

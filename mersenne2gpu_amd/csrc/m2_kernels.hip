@@ -64,38 +64,6 @@ __device__ __forceinline__ uint32_t digit31(int64_t d) {
   return d >= 0 ? mod31_u64((uint64_t)d) : neg31(mod31_u64((uint64_t)(-d)));
 }
 
-// Digit metadata of one carry segment (the 8 digits from j0), walked once and
-// kept in registers for the segment's normalise, ripple and weight steps. Per
-// digit: its weight shifts w61 (bits 0-5) and w31 (6-10), the rotations that
-// unweight it, 61 - ((w61 + ln + 1) mod 61) (11-16) and 31 - ((w31 + ln + 1)
-// mod 31) (17-21), which also remove 1/h and the centre's factor 4 (a
-// rotation by 61 or 31 is the identity), and bit 22: its width is q + 1
-// (else q).
-struct SegMeta {
-  uint32_t m[2 * SEG_ELEMS];
-};
-__device__ __forceinline__ SegMeta seg_meta(const DigitGen& D, uint32_t j0) {
-  SegMeta M;
-  DWalk w = dwalk_at(D, j0);
-#pragma unroll
-  for (int i = 0; i < 2 * SEG_ELEMS; ++i) {
-    const uint32_t w61 = dwalk_w61(w), w31 = dwalk_w31(w);
-    uint32_t a61 = w61 + D.k61;
-    a61 -= (a61 >= 61u) ? 61u : 0u;
-    uint32_t a31 = w31 + D.k31;
-    a31 -= (a31 >= 31u) ? 31u : 0u;
-    const uint32_t wide = dwalk_width(D, w) - D.q;
-    M.m[i] = w61 | (w31 << 6) | ((61u - a61) << 11) | ((31u - a31) << 17) | (wide << 22);
-    if (i + 1 < 2 * SEG_ELEMS) dwalk_next(D, w);
-  }
-  return M;
-}
-__device__ __forceinline__ uint32_t meta_w61(uint32_t m) { return m & 63u; }
-__device__ __forceinline__ uint32_t meta_w31(uint32_t m) { return (m >> 6) & 31u; }
-__device__ __forceinline__ uint32_t meta_u61(uint32_t m) { return (m >> 11) & 63u; }
-__device__ __forceinline__ uint32_t meta_u31(uint32_t m) { return (m >> 17) & 31u; }
-__device__ __forceinline__ uint32_t meta_wide(uint32_t m) { return m >> 22; }
-
 // Digit masks 2^q - 1 and 2^(q+1) - 1 of a test (uniform per test).
 struct WidthMasks {
   uint64_t q, q1;

@@ -193,6 +193,45 @@ int main() {
     CHECK(!build_class_plan(20, bad, Pw, Cw, err) && err.find("workgroup size") != std::string::npos,
           "workgroup size 256 refused: %s", err.c_str());
   }
+  // Digit walk and segment metadata against the closed forms (make_digit_gen's
+  // self-check: every digit up to n = 2^16, windows above) for random odd
+  // exponents at every transform length 2^3..2^27, plus the extremes of
+  // each bit length; the plan flag only relaxes the multi-pass width bound.
+  {
+    int lens = 0;
+    std::set<int> seen_ln;
+    for (int bits = 2; bits <= 32; ++bits) {
+      std::vector<uint32_t> cand = {(uint32_t)((1ull << (bits - 1)) | 1u), (uint32_t)((1ull << bits) - 1)};
+      for (int k = 0; k < 6; ++k) cand.push_back((uint32_t)((1ull << (bits - 1)) | (rng() & ((1ull << (bits - 1)) - 1)) | 1u));
+      // p mod n a multiple of 61, of 31 and of both: the walk's step is 0
+      // in that field and only the wraps move its counts
+      if (bits >= 8) {
+        const uint32_t p0 = cand.back();
+        const uint64_t n = 1ull << transform_log2(p0);
+        for (uint64_t f : {61ull, 31ull, 61ull * 31ull})
+          for (uint64_t k = 1; f * k < n; k += 2 + 2 * (rng() % 8)) {
+            cand.push_back((uint32_t)((p0 & ~(n - 1)) | (f * k)));
+            if (k > 16) break;
+          }
+      }
+      for (uint32_t p : cand) {
+        if (p < 3 || (p & 1u) == 0) continue;
+        const int ln = transform_log2(p);
+        if (ln > 27) continue;
+        DevPlan P{};
+        P.ln = ln;
+        P.small = true;
+        DigitGen D;
+        std::string e;
+        CHECK(make_digit_gen(p, P, D, e), "p=%u ln=%d digit walk / segment metadata: %s", p, ln, e.c_str());
+        seen_ln.insert(ln);
+        ++lens;
+      }
+    }
+    CHECK(seen_ln.size() == 25 && *seen_ln.begin() == 3 && *seen_ln.rbegin() == 27,
+          "transform lengths covered: %zu", seen_ln.size());
+    (void)lens;
+  }
   printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
   return fails ? 1 : 0;
 }
