@@ -191,16 +191,29 @@ M2_HD e61 split31(e61 w) { return {split31(w.a), split31(w.b)}; }
 
 constexpr uint64_t P61x2 = 2 * P61, P61x4 = 4 * P61, P61x8 = 8 * P61;
 
+// Values hidden from the instruction combiner (no instruction on the device):
+// 2 x1 of a limb x1 = x >> 32 is then one shift, not the funnel shift and mask
+// that (x >> 31) & ~1 becomes, and a product by the constant 2^31 stays one
+// v_mad_u64_u32 that adds into the sum instead of a 64-bit shift and two masks.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ uint32_t opaque_v(uint32_t x) { asm("" : "+v"(x)); return x; }
+__device__ __forceinline__ uint32_t opaque_s(uint32_t x) { asm("" : "+s"(x)); return x; }
+#else
+M2_HD uint32_t opaque_v(uint32_t x) { return x; }
+M2_HD uint32_t opaque_s(uint32_t x) { return x; }
+#endif
+
 // a c + b d (mod P61) in [0, P61]: a, b < 2^62; c, d in split form.
 //   a c = a0 c0 + (a0 c1 + 2 a1 c0) 2^31 + a1 c1 2^63, with 2^63 = 4 and
 //   2^61 = 1 (mod P61); the middle sum M contributes (M >> 30) + (M mod 2^30) 2^31.
 M2_HD uint64_t dot61s(uint64_t a, uint64_t c, uint64_t b, uint64_t d) {
-  const uint32_t a0 = uint32_t(a), a1 = uint32_t(a >> 32), b0 = uint32_t(b), b1 = uint32_t(b >> 32);
+  const uint32_t a0 = uint32_t(a), a1 = opaque_v(uint32_t(a >> 32)), b0 = uint32_t(b), b1 = opaque_v(uint32_t(b >> 32));
   const uint32_t c0 = uint32_t(c), c1 = uint32_t(c >> 32), d0 = uint32_t(d), d1 = uint32_t(d >> 32);
   const uint64_t lo = uint64_t(a0) * c0 + uint64_t(b0) * d0;  // each < 2^63
   const uint64_t mid = uint64_t(a0) * c1 + uint64_t(2u * a1) * c0 + uint64_t(b0) * d1 + uint64_t(2u * b1) * d0;
   const uint64_t hi = uint64_t(a1) * c1 + uint64_t(b1) * d1;  // < 2^61
-  const uint64_t r = fold61(lo) + (hi << 2) + (mid >> 30) + ((mid & 0x3FFFFFFFull) << 31);  // < 2^64
+  const uint64_t r = fold61(lo) + (hi << 2) + (mid >> 30) +
+                     uint64_t(uint32_t(mid) & 0x3FFFFFFFu) * opaque_s(0x80000000u);  // < 2^64
   return fold61(fold61(r));
 }
 // x w and x conj(w) for x with components <= 2 P61 (P61x2 - b negates b),
