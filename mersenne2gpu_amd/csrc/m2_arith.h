@@ -38,12 +38,17 @@ M2_HD uint64_t fold_product61(uint64_t p00, uint64_t p01, uint64_t p11) {
   const uint64_t r = (p11 << 3) + (p00 & P61) + (p00 >> 61) + (p01 >> 29) + ((p01 & 0x1FFFFFFFull) << 32);
   return fold61(fold61(r));
 }
-M2_HD uint64_t mul61(uint64_t x, uint64_t y) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  // one v_mad_u64_u32 per partial product
+// x y (mod P61) in [0, P61] from 32-bit limbs, one v_mad_u64_u32 per partial
+// product, for x, y < 2^62: x1, y1 < 2^30, so p11 < 2^60 and p01 < 2^63, and
+// the folded sum stays below 2^63 + 2^62 + 2^35.
+M2_HD uint64_t mul61_limbs(uint64_t x, uint64_t y) {
   const uint32_t x0 = uint32_t(x), x1 = uint32_t(x >> 32);
   const uint32_t y0 = uint32_t(y), y1 = uint32_t(y >> 32);
   return fold_product61(uint64_t(x0) * y0, uint64_t(x0) * y1 + uint64_t(x1) * y0, uint64_t(x1) * y1);
+}
+M2_HD uint64_t mul61(uint64_t x, uint64_t y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return mul61_limbs(x, y);
 #else
   unsigned __int128 t = (unsigned __int128)x * y;
   return fold61(fold61((uint64_t(t) & P61) + uint64_t(t >> 61)));
@@ -101,6 +106,10 @@ M2_HD e61 sqr(e61 x) {
   const uint64_t t = mul61(x.a, x.b);
   return {mul61(add61(x.a, x.b), sub61(x.a, x.b)), add61(t, t)};
 }
+// The same square for components <= P61, with the factors a + b, a - b
+// (as a + (P61 - b)) and 2b left unreduced: each is <= 2 P61 < 2^62, which
+// mul61 takes as it is.
+M2_HD e61 sqr_lazy(e61 x) { return {mul61(x.a + x.b, x.a + neg61(x.b)), mul61(x.a, x.b + x.b)}; }
 // The primitive 8th root of unity in this field is -(1+i) 2^30; its powers are
 // rotations plus add/sub, no multiplication.
 M2_HD e61 mulw8(e61 x) {  // * w8
@@ -385,6 +394,13 @@ M2_HD void square_pair(E& zk, E& zmk, E tw1) {
   const E C = mul_tw(sqr(u1), tw1);
   zk = sub(times4(sqr(a)), C);
   zmk = sub(times4(sqr(b)), conj(C));
+}
+M2_HD void square_pair(e61& zk, e61& zmk, e61 tw1) {
+  const e61 a = red(zk), b = red(zmk);
+  const e61 u1 = sub(a, conj(b));
+  const e61 C = mul_tw(sqr_lazy(u1), tw1);
+  zk = sub(times4(sqr_lazy(a)), C);
+  zmk = sub(times4(sqr_lazy(b)), conj(C));
 }
 
 

@@ -95,4 +95,52 @@ M2_HD uint32_t meta_u61(uint32_t m) { return 61u - ((m >> 16) & 0xFFu); }
 M2_HD uint32_t meta_u31(uint32_t m) { return 31u - ((m >> 24) & 0x7Fu); }
 M2_HD uint32_t meta_wide(uint32_t m) { return m >> 31; }
 
+// Normalisation with 64-bit words, for digit widths w = q or q + 1 with
+// 17 <= q <= 31 (so w <= 32). Such tests have no negative convolution term
+// (every digit is non-negative and the first one at least 17 bits wide), so
+// Garner's u (crt61_31) is read unsigned: with the M31 residue first made
+// canonical (P31, its second 0, gives u = P61 - 1 for the term 0 otherwise),
+// u is in [0, P61) or is P61, the second representation of 0, read as 0.
+// The term x = u P31 + t + add is u0 P31 + (t + add) plus u1 P31 2^32
+// (u = u1 2^32 + u0): two v_mad_u64_u32. The running carry is
+// kept as cA + cB 2^32; with A = u0 P31 + t + add + cA and B = u1 P31 + cB the
+// sum is A + B 2^32, its low w bits are A's (w <= 32), and the carry out is
+//   (A >> w) + (B mod 2^w) 2^(32 - w)  +  (B >> w) 2^32.
+// Bounds: cA < 2^(64-w) + 2^32 <= 2^47 + 2^32 and cB < 2^46, so A < 2^63 +
+// 2^48 and B < 2^60 + 2^46 (u0 < 2^32, u1 < 2^29, t < 2^31, add < 2^32); a
+// carry-in in [-2, 0) is cA = 2^64 - 2 or 2^64 - 1, and A, computed modulo
+// 2^64, is the true non-negative sum as long as t + add + carry-in >= 0.
+M2_HD bool norm32_applies(uint32_t q) { return q >= 17u && q <= 31u; }
+M2_HD uint32_t norm_digit32(uint64_t s, uint32_t t, uint32_t meta, uint32_t q, uint64_t add, uint64_t& cA,
+                            uint64_t& cB) {
+  uint32_t tr = rot31(t, meta_u31(meta));
+  tr = umin32(tr, tr - P31);  // P31 -> 0: u is then the unique one in [0, P61)
+  // u = (s - t)(2^31 + 1) = P31^-1 (s - t) (mod P61): with v = s + (P61 - t)
+  // <= 2 P61 (v1 < 2^30), v (2^31 + 1) = v + v0 2^31 + 4 v1 (2^63 = 4), a sum
+  // below 2^62 + 2^63 + 2^32 that two folds reduce
+  const uint64_t v = rot61(s, meta_u61(meta)) + neg61(uint64_t(tr));
+  uint64_t u = uint64_t(uint32_t(v)) * opaque_s(0x80000000u) + v;
+  u = uint64_t(uint32_t(v >> 32)) * opaque_s(4u) + u;
+  u = fold61(fold61(u));
+  u = (u == P61) ? 0ull : u;
+  const uint32_t w = q + meta_wide(meta);
+  const uint32_t mask = 0xFFFFFFFFu >> (32u - w);
+  const uint64_t A = uint64_t(uint32_t(u)) * P31 + (uint64_t(tr) + add + cA);
+  const uint64_t B = uint64_t(uint32_t(u >> 32)) * P31 + cB;
+  // (B mod 2^w) 2^(32 - w) < 2^32
+  cA = uint64_t((uint32_t(B) & mask) << (32u - w)) * opaque_s(1u) + (A >> w);
+  cB = B >> w;
+  return uint32_t(A) & mask;
+}
+// One digit of a carry ripple in the same form: the digit d (< 2^32 here)
+// plus the carry cA + cB 2^32 (cA < 2^33); returns the new digit.
+M2_HD uint32_t ripple_digit32(uint64_t d, uint32_t meta, uint32_t q, uint64_t& cA, uint64_t& cB) {
+  const uint32_t w = q + meta_wide(meta);
+  const uint32_t mask = 0xFFFFFFFFu >> (32u - w);
+  const uint64_t A = d + cA;
+  cA = uint64_t((uint32_t(cB) & mask) << (32u - w)) * opaque_s(1u) + (A >> w);
+  cB >>= w;
+  return uint32_t(A) & mask;
+}
+
 }  // namespace m2

@@ -164,9 +164,24 @@ struct SegDigits {
 // 0's is the term plus 2^width - 3 >= 0 (width >= 2), so no sum, digit or
 // carry is ever negative and the next squaring's terms stay in the range the
 // CRT reads (crt61_31). PRP tests pass ll_ones = 0 and carry-in 0.
+//
+// Tests with 17 <= q <= 31 (the same for every lane) take the 64-bit form
+// (norm_digit32); the carry-in is then 0 or -2.
 __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, int idx0, const SegMeta& M,
                                               const WidthMasks& wm, uint32_t q, i128 carry, uint64_t ll_ones,
                                               SegDigits& s) {
+  if (norm32_applies(q)) {
+    uint64_t cA = (uint64_t)(int64_t)carry, cB = 0;
+#pragma unroll
+    for (int i = 0; i < SEG_ELEMS; ++i) {
+      const e61 v = L61[PADX(idx0 + i)];
+      const e31 y = L31[PADX(idx0 + i)];
+      const uint32_t ma = M.m[2 * i], mb = M.m[2 * i + 1];
+      s.d[2 * i] = (int64_t)norm_digit32(v.a, y.a, ma, q, wm.of(ma) & ll_ones, cA, cB);
+      s.d[2 * i + 1] = (int64_t)norm_digit32(v.b, y.b, mb, q, wm.of(mb) & ll_ones, cA, cB);
+    }
+    return ((i128)cB << 32) + (i128)cA;
+  }
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
     const e61 v = L61[PADX(idx0 + i)];
@@ -184,8 +199,18 @@ __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, in
 }
 
 // Add carry c at the segment's first digit; the top digit absorbs the rest.
-// A wave skips the remaining digits once no lane has a carry left.
+// A wave skips the remaining digits once no lane has a carry left. Tests with
+// 17 <= q <= 31 have digits below 2^32 and carries in [0, 2^77) here, which
+// ripple in the 64-bit form of the normalisation (ripple_digit32).
 __device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const WidthMasks& wm, uint32_t q, i128 c) {
+  if (norm32_applies(q)) {
+    uint64_t cA = (uint64_t)c & 0xFFFFFFFFull, cB = (uint64_t)(c >> 32);
+#pragma unroll
+    for (int i = 0; i < 2 * SEG_ELEMS - 1; ++i)
+      if ((cA | cB) != 0) s.d[i] = (int64_t)ripple_digit32((uint64_t)s.d[i], M.m[i], q, cA, cB);
+    s.d[2 * SEG_ELEMS - 1] += (int64_t)(cA + (cB << 32));
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 2 * SEG_ELEMS - 1; ++i) {
     if (c != 0) {
@@ -197,15 +222,27 @@ __device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const
   s.d[2 * SEG_ELEMS - 1] += (int64_t)c;
 }
 
-// The segment's digits -> weighted field elements at LDS index idx0.
-__device__ __forceinline__ void seg_weight(const SegDigits& s, e61* L61, e31* L31, int idx0, const SegMeta& M) {
+// The segment's digits -> weighted field elements at LDS index idx0. With
+// nonneg (tests with 17 <= q <= 31) every digit is in [0, 2^61).
+template <bool NONNEG = false>
+__device__ __forceinline__ void seg_weight_as(const SegDigits& s, e61* L61, e31* L31, int idx0, const SegMeta& M) {
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
     const int64_t da = s.d[2 * i], db = s.d[2 * i + 1];
     const uint32_t ma = M.m[2 * i], mb = M.m[2 * i + 1];
-    L61[PADX(idx0 + i)] = {rot61(digit61(da), meta_w61(ma)), rot61(digit61(db), meta_w61(mb))};
-    L31[PADX(idx0 + i)] = {rot31(digit31(da), meta_w31(ma)), rot31(digit31(db), meta_w31(mb))};
+    if (NONNEG) {
+      L61[PADX(idx0 + i)] = {rot61((uint64_t)da, meta_w61(ma)), rot61((uint64_t)db, meta_w61(mb))};
+      L31[PADX(idx0 + i)] = {rot31(mod31_u64((uint64_t)da), meta_w31(ma)), rot31(mod31_u64((uint64_t)db), meta_w31(mb))};
+    } else {
+      L61[PADX(idx0 + i)] = {rot61(digit61(da), meta_w61(ma)), rot61(digit61(db), meta_w61(mb))};
+      L31[PADX(idx0 + i)] = {rot31(digit31(da), meta_w31(ma)), rot31(digit31(db), meta_w31(mb))};
+    }
   }
+}
+__device__ __forceinline__ void seg_weight(const SegDigits& s, e61* L61, e31* L31, int idx0, const SegMeta& M,
+                                           uint32_t q) {
+  if (norm32_applies(q)) seg_weight_as<true>(s, L61, L31, idx0, M);
+  else seg_weight_as<false>(s, L61, L31, idx0, M);
 }
 
 // Segment digits to / from global memory (digit pairs as int64, 64 contiguous
@@ -1007,7 +1044,7 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
   for (uint32_t it = 0; it < td.iters; ++it) {
 #pragma unroll
     for (int i = 0; i < SPT; ++i)
-      if (s0 + i < nseg) seg_weight(sd[i], L61, L31, (s0 + i) * SEG_ELEMS, M[i]);
+      if (s0 + i < nseg) seg_weight(sd[i], L61, L31, (s0 + i) * SEG_ELEMS, M[i], D.q);
     __syncthreads();
     // constant plan: steps [0, KL) from LDS, the rest (GTW) from global memory
     constexpr int NSC = CL ? small_nsteps(CL) : 0, KL = GTW ? small_lds_steps(CL, TN) : NSC;
@@ -1117,7 +1154,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fwd(const TestDesc* __rest
       seg_load(zd + (size_t)cg * (R * W) + idx0, sd);
       const SegMeta M = seg_meta(D, 2u * e0);
       if (v == 0) seg_ripple(sd, M, WidthMasks(D), D.q, carry_at(G, prev_strip_carry(cg, (uint32_t)r, R, NS)));
-      seg_weight(sd, L61, L31, idx0, M);
+      seg_weight(sd, L61, L31, idx0, M, D.q);
     }
   }
   ft.store(T61, T31, ntw);
@@ -1388,7 +1425,7 @@ __global__ void __launch_bounds__(MP_THREADS) k_outer_fused(const TestDesc* __re
     if (v == 0)
       seg_ripple(sd, M, wm, D.q,
                  carry_take(cf, prev_strip_carry(cg, r, R, NS), epoch & 0xFFu, spin_limit, sync + NS));
-    seg_weight(sd, L61, L31, idx0, M);
+    seg_weight(sd, L61, L31, idx0, M, D.q);
   }
   __syncthreads();
   if (CA) {

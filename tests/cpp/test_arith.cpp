@@ -1,12 +1,15 @@
 // CPU checks of the transform arithmetic in m2_arith.h (the same inline
 // functions the gfx950 kernels compile): split-twiddle products, the 64-bit
 // M31 reduction, the lazily reduced M61 butterflies and the centre squaring
-// formula, against plain modular arithmetic on random and extreme inputs.
+// formula, against plain modular arithmetic on random and extreme inputs; and
+// the 64-bit normalisation of a carry segment (norm_digit32) against the
+// 128-bit Garner reading, mask and shift.
 #include <stdio.h>
 
 #include <random>
 
 #include "m2_arith.h"
+#include "m2_digits.h"
 
 using namespace m2;
 typedef unsigned __int128 u128;
@@ -63,6 +66,20 @@ int main() {
     CHECK(canon61(r) == m61((u128)a * c + (u128)b * d), "dot61s a=%llx b=%llx c=%llx d=%llx",
           (unsigned long long)a, (unsigned long long)b, (unsigned long long)c, (unsigned long long)d);
     CHECK(unsplit31(split31(c)) == c, "split round trip");
+  }
+  // the limb product the kernels use: x, y < 2^62, fully reduced result
+  for (int i = 0; i < N; ++i) {
+    const uint64_t x = pick((1ull << 62) - 1), y = pick((1ull << 62) - 1);
+    const uint64_t r = mul61_limbs(x, y);
+    CHECK(r <= P61, "mul61_limbs range %llx", (unsigned long long)r);
+    CHECK(canon61(r) == m61((u128)x * y), "mul61_limbs x=%llx y=%llx", (unsigned long long)x,
+          (unsigned long long)y);
+  }
+  // squares with unreduced factors against the reduced-factor square
+  for (int i = 0; i < N; ++i) {
+    const e61 x = rnd61(P61), q = sqr_lazy(x);
+    CHECK(q.a <= P61 && q.b <= P61, "sqr_lazy range");
+    CHECK(eq61(q, sqr(x)) && eq61(q, mul(x, x)), "sqr_lazy");
   }
   // twiddle products against the plain GF(p^2) products
   for (int i = 0; i < N; ++i) {
@@ -201,6 +218,79 @@ int main() {
     CHECK(old_rule(res61(hi_term), res31(hi_term)) == hi_term - M, "old rule reads 3 2^90 + 12345 as negative");
     CHECK(crt61_31(res61(hi_term), res31(hi_term)) == hi_term, "crt61_31 reads 3 2^90 + 12345 as positive");
     CHECK(crt61_31(res61(M - 1), res31(M - 1)) == -1, "M - 1 is -1");
+  }
+  // segment normalisation with 64-bit words (17 <= q <= 31) against the
+  // 128-bit reading: 8 terms in [0, 2^92 - 2^64] with random unweighting
+  // rotations, residues in either representation of 0, PRP (add 0) and LL
+  // (add 2^w - 1, carry-in -2 at the first digit), and positive carry-ins
+  {
+    typedef __int128 i128;
+    const i128 POS_MAX = ((i128)1 << 92) - ((i128)1 << 64);
+    auto res61 = [](i128 x) { return (uint64_t)(x % (i128)P61); };
+    auto res31 = [](i128 x) { return (uint32_t)(x % (i128)P31); };
+    for (int trial = 0; trial < N / 4; ++trial) {
+      const uint32_t q = 17u + (uint32_t)(rng() % 15);
+      CHECK(norm32_applies(q), "norm32_applies %u", q);
+      const int mode = (int)(rng() % 3);  // 0 PRP, 1 LL, 2 PRP with a positive carry-in
+      const i128 cin = mode == 1 ? (rng() % 2 ? -2 : 0) : (mode == 2 ? (i128)(rng() % (1ull << 32)) : 0);
+      i128 c = cin;
+      uint64_t cA = (uint64_t)(int64_t)cin, cB = 0;
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t a61 = (uint32_t)(rng() % 61), a31 = (uint32_t)(rng() % 31), wide = (uint32_t)(rng() % 2);
+        const uint32_t meta = (a61 << 16) | (a31 << 24) | (wide << 31);
+        const uint32_t w = q + wide;
+        const int band = (int)(rng() % 4);
+        i128 x;
+        if (band == 0) x = (i128)((((u128)rng() << 64) | rng()) % (u128)(POS_MAX + 1));
+        else if (band == 1) x = POS_MAX - (i128)(rng() % (1ull << 40));
+        else if (band == 2) x = (i128)(rng() % 4) * (i128)P61 * (rng() % 2 ? 1 : (i128)P31);  // residues 0
+        else x = (i128)(rng() % (1ull << 40));
+        uint64_t s1 = res61(x);
+        uint32_t t1 = res31(x);
+        if (s1 == 0 && rng() % 2) s1 = P61;
+        if (t1 == 0 && rng() % 2) t1 = P31;
+        // the stored residues are the weighted ones: rotating them by the
+        // unweighting counts gives s1, t1
+        const uint64_t s = rot61(s1, 61u - meta_u61(meta));
+        const uint32_t t = rot31(t1, 31u - meta_u31(meta));
+        const uint64_t add = mode == 1 ? (1ull << w) - 1ull : 0ull;
+        const i128 acc = crt61_31(rot61(s, meta_u61(meta)), rot31(t, meta_u31(meta)), add) + c;
+        const uint64_t want = (uint64_t)acc & ((1ull << w) - 1ull);
+        c = acc >> w;
+        const uint32_t got = norm_digit32(s, t, meta, q, add, cA, cB);
+        CHECK(got == want, "norm_digit32 q=%u digit %d: %x against %llx", q, i, got, (unsigned long long)want);
+      }
+      const i128 cout = ((i128)cB << 32) + (i128)cA;
+      CHECK(cout == c, "norm_digit32 carry-out q=%u", q);
+    }
+    CHECK(!norm32_applies(16) && !norm32_applies(32), "norm32_applies range");
+    // the ripple of a carry in [0, 2^77) through 7 digits below 2^w, the top
+    // digit taking the rest, against 128-bit addition and shifts
+    for (int trial = 0; trial < N / 4; ++trial) {
+      const uint32_t q = 17u + (uint32_t)(rng() % 15);
+      uint64_t d[8];
+      uint32_t meta[8];
+      for (int i = 0; i < 8; ++i) {
+        meta[i] = (uint32_t)(rng() % 2) << 31;
+        const uint32_t w = q + (meta[i] >> 31);
+        d[i] = rng() % 3 == 0 ? (1ull << w) - 1ull : rng() % (1ull << w);
+      }
+      const int kind = (int)(rng() % 3);
+      i128 c = kind == 0 ? (i128)(rng() % (1ull << 40)) : (kind == 1 ? (i128)((((u128)rng() << 64) | rng()) % ((u128)1 << 77)) : 0);
+      uint64_t cA = (uint64_t)c & 0xFFFFFFFFull, cB = (uint64_t)(c >> 32);
+      i128 ref = c;
+      uint64_t got[8], want[8];
+      for (int i = 0; i < 7; ++i) {
+        const uint32_t w = q + (meta[i] >> 31);
+        const i128 acc = (i128)d[i] + ref;
+        want[i] = (uint64_t)acc & ((1ull << w) - 1ull);
+        ref = acc >> w;
+        got[i] = (cA | cB) ? ripple_digit32(d[i], meta[i], q, cA, cB) : d[i];
+      }
+      want[7] = d[7] + (uint64_t)ref;
+      got[7] = d[7] + cA + (cB << 32);
+      for (int i = 0; i < 8; ++i) CHECK(got[i] == want[i], "ripple_digit32 q=%u digit %d", q, i);
+    }
   }
   printf("%s (%d failures)\n", fails ? "FAILED" : "OK", fails);
   return fails ? 1 : 0;
