@@ -236,14 +236,24 @@ M2_HD uint32_t red31(uint64_t x) {
   const uint32_t r = uint32_t(t & P31) + uint32_t(t >> 31);
   return umin32(r, r - P31);
 }
+// x mod P31 in [0, P31] for x <= 2 P31^2 (a dot product of two reduced
+// pairs): with x = a 2^62 + b 2^31 + c (a <= 1; b, c < 2^31), x = a + b + c
+// (mod P31), and a + b + c <= 2 P31 up to that bound (a = 1 leaves b <=
+// 2^31 - 4), so one three-input add and an unsigned min reduce it, all in
+// 32-bit words.
+M2_HD uint32_t red31_dot(uint64_t x) {
+  const uint32_t y = uint32_t(x >> 31);  // bits 31..62
+  const uint32_t r = (y & P31) + (y >> 31) + (uint32_t(x) & P31);
+  return umin32(r, r - P31);
+}
 // GF(M31^2) twiddle products as two 2-term dot products each.
 M2_HD e31 mul_tw(e31 x, e31 w) {
   const uint32_t nd = P31 - w.b;
-  return {red31(uint64_t(x.a) * w.a + uint64_t(x.b) * nd), red31(uint64_t(x.a) * w.b + uint64_t(x.b) * w.a)};
+  return {red31_dot(uint64_t(x.a) * w.a + uint64_t(x.b) * nd), red31_dot(uint64_t(x.a) * w.b + uint64_t(x.b) * w.a)};
 }
 M2_HD e31 mulconj_tw(e31 x, e31 w) {
   const uint32_t nd = P31 - w.b;
-  return {red31(uint64_t(x.a) * w.a + uint64_t(x.b) * w.b), red31(uint64_t(x.b) * w.a + uint64_t(x.a) * nd)};
+  return {red31_dot(uint64_t(x.a) * w.a + uint64_t(x.b) * w.b), red31_dot(uint64_t(x.b) * w.a + uint64_t(x.a) * nd)};
 }
 
 // x 2^30 (mod P61) for any x < 2^64, result < 2^61 + 2^33.

@@ -98,6 +98,8 @@ int main() {
   for (int i = 0; i < N; ++i) {
     const uint64_t x = pick((1ull << 63) - 1);
     CHECK(red31(x) <= P31 && canon31(red31(x)) == x % P31, "red31 %llx", (unsigned long long)x);
+    const uint64_t P2 = 2ull * P31 * P31, z = i < 64 ? P2 - (uint64_t)i : pick(P2);
+    CHECK(red31_dot(z) <= P31 && canon31(red31_dot(z)) == z % P31, "red31_dot %llx", (unsigned long long)z);
     const uint64_t y = pick(~0ull);
     CHECK(rot30_lazy(y) < (1ull << 61) + (1ull << 33) && canon61(rot30_lazy(y)) == m61((u128)y << 30),
           "rot30_lazy");
