@@ -225,10 +225,14 @@ M2_HD uint64_t dot61s(uint64_t a, uint64_t c, uint64_t b, uint64_t d) {
                      uint64_t(uint32_t(mid) & 0x3FFFFFFFu) * opaque_s(0x80000000u);  // < 2^64
   return fold61(fold61(r));
 }
-// x w and x conj(w) for x with components <= 2 P61 (P61x2 - b negates b),
-// w a split twiddle; results in [0, P61].
-M2_HD e61 mul_tw(e61 x, e61 w) { return {dot61s(x.a, w.a, P61x2 - x.b, w.b), dot61s(x.a, w.b, x.b, w.a)}; }
-M2_HD e61 mulconj_tw(e61 x, e61 w) { return {dot61s(x.a, w.a, x.b, w.b), dot61s(x.b, w.a, P61x2 - x.a, w.b)}; }
+// P61 - c in split form for c <= P61 in split form: both fields complemented
+// within their widths (2^30 - 1 - c1, 2^31 - 1 - c0), i.e. two XORs.
+M2_HD uint64_t negsplit31(uint64_t w) { return w ^ 0x3FFFFFFF7FFFFFFFull; }
+// x w and x conj(w) for x with components < 2^62, w a split twiddle (the
+// negated product is that with the negated twiddle component); results in
+// [0, P61].
+M2_HD e61 mul_tw(e61 x, e61 w) { return {dot61s(x.a, w.a, x.b, negsplit31(w.b)), dot61s(x.a, w.b, x.b, w.a)}; }
+M2_HD e61 mulconj_tw(e61 x, e61 w) { return {dot61s(x.a, w.a, x.b, w.b), dot61s(x.b, w.a, x.a, negsplit31(w.b))}; }
 
 // x mod P31 in [0, P31] for x < 2^63 (2^32 = 2 mod P31).
 M2_HD uint32_t red31(uint64_t x) {
@@ -267,13 +271,13 @@ M2_HD uint64_t red61(uint64_t x) { return fold61(fold61(x)); }  // [0, 2^64) -> 
 M2_HD void dft2_fwd_lazy(e61& x0, e61& x1) {
   const e61 t = x0;
   x0 = {fold61(t.a + x1.a), fold61(t.b + x1.b)};
-  x1 = {fold61(t.a + (P61 - x1.a)), fold61(t.b + (P61 - x1.b))};
+  x1 = {fold61(t.a + neg61(x1.a)), fold61(t.b + neg61(x1.b))};
 }
 M2_HD void dft4_fwd_lazy(e61* v) {
   const uint64_t y0a = v[0].a + v[2].a, y0b = v[0].b + v[2].b;                    // <= 2P + 7
-  const uint64_t y2a = v[0].a + (P61 - v[2].a), y2b = v[0].b + (P61 - v[2].b);
+  const uint64_t y2a = v[0].a + neg61(v[2].a), y2b = v[0].b + neg61(v[2].b);
   const uint64_t y1a = v[1].a + v[3].a, y1b = v[1].b + v[3].b;                    // <= 2P
-  const uint64_t y3a = v[3].b + (P61 - v[1].b), y3b = v[1].a + (P61 - v[3].a);    // i (v1 - v3)
+  const uint64_t y3a = v[3].b + neg61(v[1].b), y3b = v[1].a + neg61(v[3].a);    // i (v1 - v3)
   v[0] = {fold61(y0a + y1a), fold61(y0b + y1b)};
   v[1] = {fold61(y0a + (P61x2 - y1a)), fold61(y0b + (P61x2 - y1b))};
   v[2] = {fold61(y2a + y3a), fold61(y2b + y3b)};
@@ -282,13 +286,13 @@ M2_HD void dft4_fwd_lazy(e61* v) {
 M2_HD void dft8_fwd_lazy(e61* v) {
   // level 1
   const uint64_t y0a = v[0].a + v[4].a, y0b = v[0].b + v[4].b;                    // <= 2P + 7
-  const uint64_t y4a = v[0].a + (P61 - v[4].a), y4b = v[0].b + (P61 - v[4].b);
+  const uint64_t y4a = v[0].a + neg61(v[4].a), y4b = v[0].b + neg61(v[4].b);
   const uint64_t y1a = v[1].a + v[5].a, y1b = v[1].b + v[5].b;                    // <= 2P
-  const uint64_t y5a = v[1].a + (P61 - v[5].a), y5b = v[1].b + (P61 - v[5].b);
+  const uint64_t y5a = v[1].a + neg61(v[5].a), y5b = v[1].b + neg61(v[5].b);
   const uint64_t y2a = v[2].a + v[6].a, y2b = v[2].b + v[6].b;
-  const uint64_t y6a = v[6].b + (P61 - v[2].b), y6b = v[2].a + (P61 - v[6].a);    // i (v2 - v6)
+  const uint64_t y6a = v[6].b + neg61(v[2].b), y6b = v[2].a + neg61(v[6].a);    // i (v2 - v6)
   const uint64_t y3a = v[3].a + v[7].a, y3b = v[3].b + v[7].b;
-  const uint64_t y7a = v[7].b + (P61 - v[3].b), y7b = v[3].a + (P61 - v[7].a);    // i (v3 - v7)
+  const uint64_t y7a = v[7].b + neg61(v[3].b), y7b = v[3].a + neg61(v[7].a);    // i (v3 - v7)
   // level 2
   const uint64_t z0a = y0a + y2a, z0b = y0b + y2b;                                // <= 4P + 7
   const uint64_t z2a = y0a + (P61x2 - y2a), z2b = y0b + (P61x2 - y2b);
@@ -315,13 +319,13 @@ M2_HD void dft8_fwd_lazy(e61* v) {
 M2_HD void dft2_inv_lazy(e61& x0, e61& x1) {
   const e61 t = x0;
   x0 = {fold61(t.a + x1.a), fold61(t.b + x1.b)};
-  x1 = {fold61(t.a + (P61 - x1.a)), fold61(t.b + (P61 - x1.b))};
+  x1 = {fold61(t.a + neg61(x1.a)), fold61(t.b + neg61(x1.b))};
 }
 M2_HD void dft4_inv_lazy(e61* v) {
   const uint64_t z0a = v[0].a + v[1].a, z0b = v[0].b + v[1].b;                    // <= 2P
-  const uint64_t z1a = v[0].a + (P61 - v[1].a), z1b = v[0].b + (P61 - v[1].b);
+  const uint64_t z1a = v[0].a + neg61(v[1].a), z1b = v[0].b + neg61(v[1].b);
   const uint64_t z2a = v[2].a + v[3].a, z2b = v[2].b + v[3].b;
-  const uint64_t z3a = v[2].b + (P61 - v[3].b), z3b = v[3].a + (P61 - v[2].a);    // -i (v2 - v3)
+  const uint64_t z3a = v[2].b + neg61(v[3].b), z3b = v[3].a + neg61(v[2].a);    // -i (v2 - v3)
   v[0] = {red61(z0a + z2a), red61(z0b + z2b)};
   v[2] = {fold61(z0a + (P61x2 - z2a)), fold61(z0b + (P61x2 - z2b))};
   v[1] = {fold61(z1a + z3a), fold61(z1b + z3b)};
@@ -330,14 +334,14 @@ M2_HD void dft4_inv_lazy(e61* v) {
 M2_HD void dft8_inv_lazy(e61* v) {
   // level 1
   const uint64_t z0a = v[0].a + v[1].a, z0b = v[0].b + v[1].b;                    // <= 2P
-  const uint64_t z1a = v[0].a + (P61 - v[1].a), z1b = v[0].b + (P61 - v[1].b);
+  const uint64_t z1a = v[0].a + neg61(v[1].a), z1b = v[0].b + neg61(v[1].b);
   const uint64_t z2a = v[2].a + v[3].a, z2b = v[2].b + v[3].b;
-  const uint64_t z3a = v[2].b + (P61 - v[3].b), z3b = v[3].a + (P61 - v[2].a);    // -i (v2 - v3)
+  const uint64_t z3a = v[2].b + neg61(v[3].b), z3b = v[3].a + neg61(v[2].a);    // -i (v2 - v3)
   const uint64_t z4a = v[4].a + v[5].a, z4b = v[4].b + v[5].b;
-  const uint64_t da = v[4].a + (P61 - v[5].a), db = v[4].b + (P61 - v[5].b);
+  const uint64_t da = v[4].a + neg61(v[5].a), db = v[4].b + neg61(v[5].b);
   const uint64_t z5a = rot30_lazy(P61x4 - da - db), z5b = rot30_lazy(da + (P61x2 - db));  // conj(w8) d
   const uint64_t z6a = v[6].a + v[7].a, z6b = v[6].b + v[7].b;
-  const uint64_t ma = v[6].b + (P61 - v[7].b), mb = v[7].a + (P61 - v[6].a);      // -i (v6 - v7)
+  const uint64_t ma = v[6].b + neg61(v[7].b), mb = v[7].a + neg61(v[6].a);      // -i (v6 - v7)
   const uint64_t z7a = rot30_lazy(P61x4 - ma - mb), z7b = rot30_lazy(ma + (P61x2 - mb));
   // level 2 (z5, z7 < 2^61 + 2^33)
   const uint64_t y0a = z0a + z2a, y0b = z0b + z2b;                                // <= 4P
