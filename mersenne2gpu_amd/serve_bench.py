@@ -11,14 +11,19 @@ whenever the previous one ends.
 
 Reported: p50/p99 completion latency (completion - arrival) and tests/s,
 beside the plain form: the same arrivals served one test at a time in
-arrival order (a FIFO queue) with each test taking its iterations times its
-length's measured single-test µs/iteration.
+arrival order (a FIFO queue). With --plain measured (the default) that form
+is run: each request, once it has arrived and the one before it is done, is
+an m2_test created, run and read alone, timed by the wall clock; its tests/s
+at capacity is the count over the sum of those service times. --plain model
+instead gives each test its iterations times its length's measured
+single-test µs/iteration.
 
 Parity: every completion whose (p, mode, iterations) is in
-tests/golden/c5.json is compared with the exact value, and a seeded sample of
-requests is re-run alone (m2_test) and compared with its in-mix Res64.
+tests/golden/c5.json is compared with the exact value; the measured plain
+form re-runs every request alone and compares its Res64 with the in-mix one
+(the model form re-runs a seeded sample).
 
-    python -m mersenne2gpu_amd.serve_bench [--count 2000] [--rate 40] [--iters 2000]
+    python -m mersenne2gpu_amd.serve_bench [--count 2000] [--rate 40] [--iters 2000] [--plain measured]
 """
 from __future__ import annotations
 
@@ -61,6 +66,8 @@ def main(argv=None) -> int:
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--slice", type=int, default=1000)
     ap.add_argument("--alone-sample", type=int, default=48)
+    ap.add_argument("--plain", choices=("measured", "model"), default="measured",
+                    help="run the one-at-a-time form (measured) or price it from per-length rates (model)")
     args = ap.parse_args(argv)
 
     reqs = W.serve_stream(args.seed, args.count, args.rate, args.iters)
@@ -69,7 +76,7 @@ def main(argv=None) -> int:
         len(reqs), reqs[-1][0], {ln: lns.count(ln) for ln in sorted(set(lns))}))
 
     ctx = m2.Context(0)
-    rates = plain_rates(ctx, lns)
+    rates = plain_rates(ctx, lns) if args.plain == "model" else None
 
     srv = m2.Server(ctx, hbm_budget_bytes=int(args.budget_gb * 1e9), slice_iters=args.slice)
     ids = {}
@@ -100,9 +107,29 @@ def main(argv=None) -> int:
     log("served %d requests in %.3f s (peak %d active, %.2f GB)" % (len(reqs), makespan, peak_active,
                                                                    peak_bytes / 1e9))
 
-    # plain form: FIFO, one test at a time, measured per-length rates
-    service = [k * rates[ln] * 1e-6 for (_, _, _, k), ln in zip(reqs, lns)]
-    plain_lat, plain_end = fifo_latencies(reqs, service)
+    alone = None
+    if args.plain == "measured":
+        # plain form, run: FIFO in arrival order, one m2_test at a time
+        plain_lat, service, alone = [], [], []
+        t0 = time.perf_counter()
+        for i, (arr, p, mode, k) in enumerate(reqs):
+            wait = arr - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            s0 = time.perf_counter()
+            with m2.Test(ctx, p, mode) as t:
+                t.run(k)
+                alone.append(t.res64())
+            service.append(time.perf_counter() - s0)
+            plain_lat.append(time.perf_counter() - t0 - arr)
+            if (i + 1) % 250 == 0:
+                log("plain form: %d of %d requests done at %.1f s" % (i + 1, len(reqs), time.perf_counter() - t0))
+        plain_end = time.perf_counter() - t0
+        log("plain form run: %d requests in %.3f s (service %.3f s)" % (len(reqs), plain_end, sum(service)))
+    else:
+        # plain form, priced: FIFO, one test at a time, measured per-length rates
+        service = [k * rates[ln] * 1e-6 for (_, _, _, k), ln in zip(reqs, lns)]
+        plain_lat, plain_end = fifo_latencies(reqs, service)
 
     # parity: exact fixture, and a sample re-run alone
     fx = load_fixture("c5.json")
@@ -115,16 +142,23 @@ def main(argv=None) -> int:
                 if done[i][1] != want[(p, mode, k)]:
                     mismatched += 1
                     log("MISMATCH (fixture) p=%d %s k=%d" % (p, mode, k))
-    rnd = random.Random(args.seed)
-    sample = rnd.sample(range(len(reqs)), min(args.alone_sample, len(reqs)))
     alone_mismatch = 0
-    for i in sample:
-        _, p, mode, k = reqs[i]
-        with m2.Test(ctx, p, mode) as t:
-            t.run(k)
-            if t.res64() != done[i][1]:
+    if alone is not None:
+        sample = list(range(len(reqs)))
+        for i in sample:
+            if alone[i] != done[i][1]:
                 alone_mismatch += 1
-                log("MISMATCH (alone) p=%d %s k=%d" % (p, mode, k))
+                log("MISMATCH (alone) p=%d %s k=%d" % reqs[i][1:])
+    else:
+        rnd = random.Random(args.seed)
+        sample = rnd.sample(range(len(reqs)), min(args.alone_sample, len(reqs)))
+        for i in sample:
+            _, p, mode, k = reqs[i]
+            with m2.Test(ctx, p, mode) as t:
+                t.run(k)
+                if t.res64() != done[i][1]:
+                    alone_mismatch += 1
+                    log("MISMATCH (alone) p=%d %s k=%d" % (p, mode, k))
     ctx.close()
 
     out = {
@@ -137,9 +171,12 @@ def main(argv=None) -> int:
         "makespan_s": round(makespan, 3),
         "plain": {"tests_per_s": round(len(reqs) / plain_end, 3), "p50_s": round(pct(plain_lat, 50), 4),
                   "p99_s": round(pct(plain_lat, 99), 4), "makespan_s": round(plain_end, 3),
-                  "how": "FIFO one test at a time, measured single-test us/iter per length"},
+                  "service_s": round(sum(service), 3),
+                  "how": ("measured: FIFO in arrival order, one m2_test at a time" if alone is not None else
+                          "FIFO one test at a time, measured single-test us/iter per length")},
         "p99_vs_plain": round(pct(plain_lat, 99) / pct(lat, 99), 2),
         "tests_per_s_vs_plain": round(plain_end / makespan, 2),
+        "capacity_vs_plain_service": round(sum(service) / makespan, 2),
         "peak_active": peak_active,
         "peak_bytes": peak_bytes,
         "res64_checked_fixture": checked,
