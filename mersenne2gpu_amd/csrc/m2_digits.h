@@ -132,6 +132,34 @@ M2_HD uint32_t norm_digit32(uint64_t s, uint32_t t, uint32_t meta, uint32_t q, u
   cB = B >> w;
   return uint32_t(A) & mask;
 }
+// The same for digit widths w = q or q + 1 with 32 <= q <= 43 (w = 32..44),
+// with the carry in one word: with A = u0 P31 + t + add + c (< 2^63 + 2^61 +
+// 2^45) and C = (A >> 32) + u1 P31 (< 2^61) the sum is (A mod 2^32) + C 2^32,
+// its low w bits are A's low word and C's low w - 32 bits above it, and the
+// carry out is C >> (w - 32) (< 2^61). LL's addend 2^w - 1 is below 2^44 here.
+M2_HD bool norm64_applies(uint32_t q) { return q >= 32u && q <= 43u; }
+M2_HD uint64_t norm_digit64(uint64_t s, uint32_t t, uint32_t meta, uint32_t q, uint64_t add, uint64_t& c) {
+  uint32_t tr = rot31(t, meta_u31(meta));
+  tr = umin32(tr, tr - P31);
+  const uint64_t v = rot61(s, meta_u61(meta)) + neg61(uint64_t(tr));
+  uint64_t u = uint64_t(uint32_t(v)) * opaque_s(0x80000000u) + v;
+  u = uint64_t(uint32_t(v >> 32)) * opaque_s(4u) + u;
+  u = fold61(fold61(u));
+  u = (u == P61) ? 0ull : u;
+  const uint32_t ws = q + meta_wide(meta) - 32u;  // 0..12
+  const uint64_t A = uint64_t(uint32_t(u)) * P31 + (uint64_t(tr) + add + c);
+  const uint64_t C = uint64_t(uint32_t(u >> 32)) * P31 + (A >> 32);
+  c = C >> ws;
+  return (uint64_t(uint32_t(C) & ~(0xFFFFFFFFu << ws)) << 32) | uint32_t(A);
+}
+// One digit of a carry ripple for widths 32..44: digit d < 2^w, carry c <
+// 2^61 in one word.
+M2_HD uint64_t ripple_digit64(uint64_t d, uint32_t meta, uint32_t q, uint64_t& c) {
+  const uint32_t w = q + meta_wide(meta);
+  const uint64_t A = d + c;
+  c = A >> w;
+  return A & ((uint64_t(1) << w) - 1u);
+}
 // One digit of a carry ripple in the same form: the digit d (< 2^32 here)
 // plus the carry cA + cB 2^32 (cA < 2^33); returns the new digit.
 M2_HD uint32_t ripple_digit32(uint64_t d, uint32_t meta, uint32_t q, uint64_t& cA, uint64_t& cB) {

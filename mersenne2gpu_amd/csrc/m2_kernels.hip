@@ -165,8 +165,8 @@ struct SegDigits {
 // carry is ever negative and the next squaring's terms stay in the range the
 // CRT reads (crt61_31). PRP tests pass ll_ones = 0 and carry-in 0.
 //
-// Tests with 17 <= q <= 31 (the same for every lane) take the 64-bit form
-// (norm_digit32); the carry-in is then 0 or -2.
+// Tests with 17 <= q <= 31 or 32 <= q <= 43 (the same for every lane) take
+// the 64-bit forms (norm_digit32, norm_digit64); the carry-in is then 0 or -2.
 __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, int idx0, const SegMeta& M,
                                               const WidthMasks& wm, uint32_t q, i128 carry, uint64_t ll_ones,
                                               SegDigits& s) {
@@ -181,6 +181,18 @@ __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, in
       s.d[2 * i + 1] = (int64_t)norm_digit32(v.b, y.b, mb, q, wm.of(mb) & ll_ones, cA, cB);
     }
     return ((i128)cB << 32) + (i128)cA;
+  }
+  if (norm64_applies(q)) {
+    uint64_t c = (uint64_t)(int64_t)carry;
+#pragma unroll
+    for (int i = 0; i < SEG_ELEMS; ++i) {
+      const e61 v = L61[PADX(idx0 + i)];
+      const e31 y = L31[PADX(idx0 + i)];
+      const uint32_t ma = M.m[2 * i], mb = M.m[2 * i + 1];
+      s.d[2 * i] = (int64_t)norm_digit64(v.a, y.a, ma, q, wm.of(ma) & ll_ones, c);
+      s.d[2 * i + 1] = (int64_t)norm_digit64(v.b, y.b, mb, q, wm.of(mb) & ll_ones, c);
+    }
+    return (i128)c;
   }
 #pragma unroll
   for (int i = 0; i < SEG_ELEMS; ++i) {
@@ -201,7 +213,8 @@ __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, in
 // Add carry c at the segment's first digit; the top digit absorbs the rest.
 // A wave skips the remaining digits once no lane has a carry left. Tests with
 // 17 <= q <= 31 have digits below 2^32 and carries in [0, 2^77) here, which
-// ripple in the 64-bit form of the normalisation (ripple_digit32).
+// ripple in the 64-bit form of the normalisation (ripple_digit32); with
+// 32 <= q <= 43 carries are below 2^61 (ripple_digit64).
 __device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const WidthMasks& wm, uint32_t q, i128 c) {
   if (norm32_applies(q)) {
     uint64_t cA = (uint64_t)c & 0xFFFFFFFFull, cB = (uint64_t)(c >> 32);
@@ -209,6 +222,14 @@ __device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const
     for (int i = 0; i < 2 * SEG_ELEMS - 1; ++i)
       if ((cA | cB) != 0) s.d[i] = (int64_t)ripple_digit32((uint64_t)s.d[i], M.m[i], q, cA, cB);
     s.d[2 * SEG_ELEMS - 1] += (int64_t)(cA + (cB << 32));
+    return;
+  }
+  if (norm64_applies(q)) {  // carries in [0, 2^61)
+    uint64_t cc = (uint64_t)c;
+#pragma unroll
+    for (int i = 0; i < 2 * SEG_ELEMS - 1; ++i)
+      if (cc != 0) s.d[i] = (int64_t)ripple_digit64((uint64_t)s.d[i], M.m[i], q, cc);
+    s.d[2 * SEG_ELEMS - 1] += (int64_t)cc;
     return;
   }
 #pragma unroll
@@ -223,7 +244,7 @@ __device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const
 }
 
 // The segment's digits -> weighted field elements at LDS index idx0. With
-// nonneg (tests with 17 <= q <= 31) every digit is in [0, 2^61).
+// nonneg (tests with 17 <= q <= 43) every digit is in [0, 2^61).
 template <bool NONNEG = false>
 __device__ __forceinline__ void seg_weight_as(const SegDigits& s, e61* L61, e31* L31, int idx0, const SegMeta& M) {
 #pragma unroll
@@ -241,7 +262,7 @@ __device__ __forceinline__ void seg_weight_as(const SegDigits& s, e61* L61, e31*
 }
 __device__ __forceinline__ void seg_weight(const SegDigits& s, e61* L61, e31* L31, int idx0, const SegMeta& M,
                                            uint32_t q) {
-  if (norm32_applies(q)) seg_weight_as<true>(s, L61, L31, idx0, M);
+  if (norm32_applies(q) || norm64_applies(q)) seg_weight_as<true>(s, L61, L31, idx0, M);
   else seg_weight_as<false>(s, L61, L31, idx0, M);
 }
 

@@ -71,8 +71,9 @@ M2_FLOOR_PROBE(sq_31, e31, 2, true, square_pair(v[0], v[1], w[0]))
 // Carry of one segment (8 digits): spectrum in LDS, digits and digit
 // metadata as in the outer passes. MODE 0: skeleton, 1: unweight + CRT +
 // normalise, 2: ripple + weight, 3: both, 4: weight alone; every part takes
-// its form for digit widths 17..32 (q = 26, as in both workloads) unless MODE
-// has bit 8, the general one (q = 40).
+// its form for digit widths 17..32 (q = 26, as in both workloads), with bit 8
+// of MODE the one for widths 32..44 (q = 40), with bit 16 the general one
+// (q = 50).
 template <int MODE>
 __device__ void carry_probe(const TestDesc* tests, e61* zo) {
   __shared__ e61 L61[padded(PASS_MAX_ELEMS)];
@@ -90,7 +91,7 @@ __device__ void carry_probe(const TestDesc* tests, e61* zo) {
   const WidthMasks wm(td.dgen);
   const e61 cw = zo[t + 300000];
   i128 c = (i128)(((unsigned __int128)cw.b << 64) | cw.a);
-  const uint32_t q = (MODE & 8) ? 40u : 26u;
+  const uint32_t q = (MODE & 16) ? 50u : ((MODE & 8) ? 40u : 26u);
   if (MODE & 1) c = seg_normalise(L61, L31, idx0, M, wm, q, c, 0ull, sd);
   if (MODE & 2) seg_ripple(sd, M, wm, q, c);
   if (MODE & 6) seg_weight(sd, L61, L31, idx0, M, q);
@@ -105,7 +106,8 @@ __device__ void carry_probe(const TestDesc* tests, e61* zo) {
 }
 extern "C" __global__ void __launch_bounds__(256) skel_carry(const TestDesc* tests, e61* zo) { carry_probe<0>(tests, zo); }
 extern "C" __global__ void __launch_bounds__(256) carry_norm(const TestDesc* tests, e61* zo) { carry_probe<1>(tests, zo); }
-extern "C" __global__ void __launch_bounds__(256) carry_norm128(const TestDesc* tests, e61* zo) { carry_probe<9>(tests, zo); }
+extern "C" __global__ void __launch_bounds__(256) carry_norm64(const TestDesc* tests, e61* zo) { carry_probe<9>(tests, zo); }
+extern "C" __global__ void __launch_bounds__(256) carry_norm128(const TestDesc* tests, e61* zo) { carry_probe<17>(tests, zo); }
 extern "C" __global__ void __launch_bounds__(256) carry_rw(const TestDesc* tests, e61* zo) { carry_probe<2>(tests, zo); }
 extern "C" __global__ void __launch_bounds__(256) carry_all(const TestDesc* tests, e61* zo) { carry_probe<3>(tests, zo); }
 extern "C" __global__ void __launch_bounds__(256) carry_w(const TestDesc* tests, e61* zo) { carry_probe<4>(tests, zo); }

@@ -6,8 +6,9 @@ Compiles csrc/probes/valu_floor.hip for gfx950 (hipcc, no GPU needed) and
 counts the VALU instructions of each arithmetic primitive in the assembly: a
 radix-r butterfly with and without twiddle products in either field, the
 squaring of a spectral pair, and the carry of one 8-digit segment (unweight,
-CRT and normalisation, in the 64-bit form that digit widths 17..32 take and,
-as norm128, the 128-bit form of the others; ripple; weight), each minus a skeleton kernel with the
+CRT and normalisation, in the 64-bit form that digit widths 17..32 take, as
+norm64 the one of widths 32..44 and as norm128 the 128-bit form of the
+others; ripple; weight), each minus a skeleton kernel with the
 same loads and stores. A kernel's floor per wave is the sum of the
 primitives each of its threads runs (half the waves of every kernel work on
 the M61 field and half on the M31 field; every thread carries one segment),
@@ -93,6 +94,7 @@ def primitives(asm: str) -> dict:
         out["sq_" + f] = c("sq_" + f) - c("skel2_" + f)
     sk = c("skel_carry")
     out["norm"] = c("carry_norm") - sk
+    out["norm64"] = c("carry_norm64") - sk
     out["norm128"] = c("carry_norm128") - sk
     out["weight"] = c("carry_w") - sk
     out["ripple"] = c("carry_rw") - c("carry_w")

@@ -266,6 +266,55 @@ int main() {
       CHECK(cout == c, "norm_digit32 carry-out q=%u", q);
     }
     CHECK(!norm32_applies(16) && !norm32_applies(32), "norm32_applies range");
+    CHECK(norm64_applies(32) && norm64_applies(43) && !norm64_applies(31) && !norm64_applies(44),
+          "norm64_applies range");
+    // widths 32..44 (q = 32..43): the one-word carry against the 128-bit reading
+    for (int trial = 0; trial < N / 4; ++trial) {
+      const uint32_t q = 32u + (uint32_t)(rng() % 12);
+      const int mode = (int)(rng() % 3);
+      const i128 cin = mode == 1 ? (rng() % 2 ? -2 : 0) : (mode == 2 ? (i128)(rng() % (1ull << 60)) : 0);
+      i128 c = cin;
+      uint64_t cw = (uint64_t)(int64_t)cin;
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t a61 = (uint32_t)(rng() % 61), a31 = (uint32_t)(rng() % 31), wide = (uint32_t)(rng() % 2);
+        const uint32_t meta = (a61 << 16) | (a31 << 24) | (wide << 31);
+        const uint32_t w = q + wide;
+        const int band = (int)(rng() % 4);
+        i128 x;
+        if (band == 0) x = (i128)((((u128)rng() << 64) | rng()) % (u128)(POS_MAX + 1));
+        else if (band == 1) x = POS_MAX - (i128)(rng() % (1ull << 40));
+        else if (band == 2) x = (i128)(rng() % 4) * (i128)P61 * (rng() % 2 ? 1 : (i128)P31);
+        else x = (i128)(rng() % (1ull << 40));
+        uint64_t s1 = res61(x);
+        uint32_t t1 = res31(x);
+        if (s1 == 0 && rng() % 2) s1 = P61;
+        if (t1 == 0 && rng() % 2) t1 = P31;
+        const uint64_t s = rot61(s1, 61u - meta_u61(meta));
+        const uint32_t t = rot31(t1, 31u - meta_u31(meta));
+        const uint64_t add = mode == 1 ? (1ull << w) - 1ull : 0ull;
+        const i128 acc = crt61_31(rot61(s, meta_u61(meta)), rot31(t, meta_u31(meta)), add) + c;
+        const uint64_t want = (uint64_t)acc & ((1ull << w) - 1ull);
+        c = acc >> w;
+        const uint64_t got = norm_digit64(s, t, meta, q, add, cw);
+        CHECK(got == want, "norm_digit64 q=%u digit %d: %llx against %llx", q, i, (unsigned long long)got,
+              (unsigned long long)want);
+      }
+      CHECK((i128)cw == c, "norm_digit64 carry-out q=%u", q);
+      // ripple of the carry-out through 7 digits below 2^w, the top digit taking the rest
+      uint64_t d[8], got[8], want[8];
+      for (int i = 0; i < 8; ++i) d[i] = rng() % 3 == 0 ? (1ull << q) - 1ull : rng() % (1ull << q);
+      i128 ref = c;
+      uint64_t cr = cw;
+      for (int i = 0; i < 7; ++i) {
+        const i128 a = (i128)d[i] + ref;
+        want[i] = (uint64_t)a & ((1ull << q) - 1ull);
+        ref = a >> q;
+        got[i] = cr ? ripple_digit64(d[i], 0u, q, cr) : d[i];
+      }
+      want[7] = d[7] + (uint64_t)ref;
+      got[7] = d[7] + cr;
+      for (int i = 0; i < 8; ++i) CHECK(got[i] == want[i], "ripple_digit64 q=%u digit %d", q, i);
+    }
     // the ripple of a carry in [0, 2^77) through 7 digits below 2^w, the top
     // digit taking the rest, against 128-bit addition and shifts
     for (int trial = 0; trial < N / 4; ++trial) {
