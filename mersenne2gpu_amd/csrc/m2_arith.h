@@ -315,7 +315,7 @@ M2_HD void dft8_fwd_lazy(e61* v) {
   v[7] = {fold61(z6a + (P61x2 - z7a)), fold61(z6b + (P61x2 - z7b))};
 }
 // Inverse butterflies: inputs <= P; output 0 fully reduced (<= P), the others
-// folded once (< 2^62, ready for mulconj_tw).
+// folded once (<= P + 7, ready for mulconj_tw; one more fold reduces them).
 M2_HD void dft2_inv_lazy(e61& x0, e61& x1) {
   const e61 t = x0;
   x0 = {fold61(t.a + x1.a), fold61(t.b + x1.b)};

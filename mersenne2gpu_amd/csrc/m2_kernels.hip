@@ -443,8 +443,10 @@ __device__ __forceinline__ void lds_step(e61* L61, e31* L31, int logR, int logW,
 #pragma unroll
           for (int e = 1; e < RADIX; ++e) v[e] = mulconj_tw(v[e], T61[e - 1]);
         } else {
+          // the lazy inverse outputs are folded once (<= P61 + 7; output 0 of
+          // radix 4 and 8 already reduced), so one more fold reduces them
 #pragma unroll
-          for (int e = 0; e < RADIX; ++e) v[e] = {red61(v[e].a), red61(v[e].b)};
+          for (int e = RADIX == 2 ? 0 : 1; e < RADIX; ++e) v[e] = {fold61(v[e].a), fold61(v[e].b)};
         }
       }
 #pragma unroll

@@ -44,13 +44,13 @@ M2_FLOOR_SKEL(skel2_61, e61, 2)
 M2_FLOOR_PROBE(fwd8t_61, e61, 8, true, for (int e = 1; e < 8; ++e) v[e] = mul_tw(v[e], w[e - 1]); dft8_fwd_lazy(v))
 M2_FLOOR_PROBE(inv8t_61, e61, 8, true, dft8_inv_lazy(v); for (int e = 1; e < 8; ++e) v[e] = mulconj_tw(v[e], w[e - 1]))
 M2_FLOOR_PROBE(fwd8u_61, e61, 8, true, dft8_fwd_lazy(v))
-M2_FLOOR_PROBE(inv8u_61, e61, 8, true, dft8_inv_lazy(v); for (int e = 0; e < 8; ++e) v[e] = {red61(v[e].a), red61(v[e].b)})
+M2_FLOOR_PROBE(inv8u_61, e61, 8, true, dft8_inv_lazy(v); for (int e = 1; e < 8; ++e) v[e] = {fold61(v[e].a), fold61(v[e].b)})
 M2_FLOOR_PROBE(fwd4t_61, e61, 4, true, for (int e = 1; e < 4; ++e) v[e] = mul_tw(v[e], w[e - 1]); dft4_fwd_lazy(v))
 M2_FLOOR_PROBE(inv4t_61, e61, 4, true, dft4_inv_lazy(v); for (int e = 1; e < 4; ++e) v[e] = mulconj_tw(v[e], w[e - 1]))
 M2_FLOOR_PROBE(fwd2t_61, e61, 2, true, v[1] = mul_tw(v[1], w[0]); dft2_fwd_lazy(v[0], v[1]))
 M2_FLOOR_PROBE(inv2t_61, e61, 2, true, dft2_inv_lazy(v[0], v[1]); v[1] = mulconj_tw(v[1], w[0]))
 M2_FLOOR_PROBE(fwd2u_61, e61, 2, true, dft2_fwd_lazy(v[0], v[1]))
-M2_FLOOR_PROBE(inv2u_61, e61, 2, true, dft2_inv_lazy(v[0], v[1]); for (int e = 0; e < 2; ++e) v[e] = {red61(v[e].a), red61(v[e].b)})
+M2_FLOOR_PROBE(inv2u_61, e61, 2, true, dft2_inv_lazy(v[0], v[1]); for (int e = 0; e < 2; ++e) v[e] = {fold61(v[e].a), fold61(v[e].b)})
 M2_FLOOR_PROBE(sq_61, e61, 2, true, square_pair(v[0], v[1], w[0]))
 
 M2_FLOOR_SKEL(skel8_31, e31, 8)

@@ -149,11 +149,12 @@ int main() {
     CHECK(v[0].a <= P61 && v[0].b <= P61 && v4[0].a <= P61 && v4[0].b <= P61, "inv output 0 reduced");
     CHECK(v2[0].a <= P61 && v2[1].a <= P61 && v2[0].b <= P61 && v2[1].b <= P61, "inv2 reduced");
     for (int e = 0; e < 8; ++e) {
-      CHECK(v[e].a < (1ull << 62) && v[e].b < (1ull << 62), "inv8 range");
+      CHECK(v[e].a <= P61 + 7 && v[e].b <= P61 + 7, "inv8 range");
+      CHECK(fold61(v[e].a) <= P61 && fold61(v[e].b) <= P61, "inv8 one fold reduces");
       CHECK(eq61(v[e], ref[e]), "dft8_inv_lazy e=%d", e);
     }
-    for (int e = 0; e < 4; ++e) CHECK(eq61(v4[e], r4[e]), "dft4_inv_lazy e=%d", e);
-    for (int e = 0; e < 2; ++e) CHECK(eq61(v2[e], r2[e]), "dft2_inv_lazy e=%d", e);
+    for (int e = 0; e < 4; ++e) CHECK(eq61(v4[e], r4[e]) && v4[e].a <= P61 + 7 && v4[e].b <= P61 + 7, "dft4_inv_lazy e=%d", e);
+    for (int e = 0; e < 2; ++e) CHECK(eq61(v2[e], r2[e]) && v2[e].a <= P61 + 7 && v2[e].b <= P61 + 7, "dft2_inv_lazy e=%d", e);
   }
   // centre squaring: 4 z_k^2 - C, 4 z_mk^2 - conj(C) equals the defining
   // formula v0 +/- v1 with u0 = z_k + conj(z_mk), u1 = z_k - conj(z_mk)
