@@ -96,7 +96,7 @@ M2_HD uint32_t meta_u31(uint32_t m) { return 31u - ((m >> 24) & 0x7Fu); }
 M2_HD uint32_t meta_wide(uint32_t m) { return m >> 31; }
 
 // Normalisation with 64-bit words, for digit widths w = q or q + 1 with
-// 17 <= q <= 31 (so w <= 32). Such tests have no negative convolution term
+// 16 <= q <= 31 (so w <= 32). Such tests have no negative convolution term
 // (every digit is non-negative and the first one at least 17 bits wide), so
 // Garner's u (crt61_31) is read unsigned: with the M31 residue first made
 // canonical (P31, its second 0, gives u = P61 - 1 for the term 0 otherwise),
@@ -106,11 +106,11 @@ M2_HD uint32_t meta_wide(uint32_t m) { return m >> 31; }
 // kept as cA + cB 2^32; with A = u0 P31 + t + add + cA and B = u1 P31 + cB the
 // sum is A + B 2^32, its low w bits are A's (w <= 32), and the carry out is
 //   (A >> w) + (B mod 2^w) 2^(32 - w)  +  (B >> w) 2^32.
-// Bounds: cA < 2^(64-w) + 2^32 <= 2^47 + 2^32 and cB < 2^46, so A < 2^63 +
-// 2^48 and B < 2^60 + 2^46 (u0 < 2^32, u1 < 2^29, t < 2^31, add < 2^32); a
+// Bounds: cA < 2^(64-w) + 2^32 <= 2^48 + 2^32 and cB < 2^47, so A < 2^63 +
+// 2^49 and B < 2^60 + 2^47 (u0 < 2^32, u1 < 2^29, t < 2^31, add < 2^32); a
 // carry-in in [-2, 0) is cA = 2^64 - 2 or 2^64 - 1, and A, computed modulo
 // 2^64, is the true non-negative sum as long as t + add + carry-in >= 0.
-M2_HD bool norm32_applies(uint32_t q) { return q >= 17u && q <= 31u; }
+M2_HD bool norm32_applies(uint32_t q) { return q >= 16u && q <= 31u; }
 M2_HD uint32_t norm_digit32(uint64_t s, uint32_t t, uint32_t meta, uint32_t q, uint64_t add, uint64_t& cA,
                             uint64_t& cB) {
   uint32_t tr = rot31(t, meta_u31(meta));

@@ -312,6 +312,12 @@ bool make_digit_gen(uint32_t p, const DevPlan& P, DigitGen& D, std::string& err)
     err = "plan error: digit width below 12 bits on the multi-pass plan";
     return false;
   }
+  // The single-workgroup kernels compiled for n >= 16 carry in 64-bit words
+  // only (16 <= q <= 43); every exponent of those lengths has 16 <= q <= 42.
+  if (ln >= 4 && (q < 16 || q > 43)) {
+    err = "plan error: digit width outside 16..44 bits at n >= 16";
+    return false;
+  }
   // Digit widths ceil(p(j+1)/n) - ceil(pj/n) and weight shifts
   // lr*(n - (pj mod n)) with 2^lr an n-th root of 2 in each field
   // (mersenne2_gpu.cpp:3921-3944).

@@ -165,8 +165,11 @@ struct SegDigits {
 // carry is ever negative and the next squaring's terms stay in the range the
 // CRT reads (crt61_31). PRP tests pass ll_ones = 0 and carry-in 0.
 //
-// Tests with 17 <= q <= 31 or 32 <= q <= 43 (the same for every lane) take
+// Tests with 16 <= q <= 31 or 32 <= q <= 43 (the same for every lane) take
 // the 64-bit forms (norm_digit32, norm_digit64); the carry-in is then 0 or -2.
+// GENERAL = false leaves out the 128-bit form of other widths: transform
+// lengths n >= 16 only have exponents with 16 <= q <= 42 (build_plan checks).
+template <bool GENERAL = true>
 __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, int idx0, const SegMeta& M,
                                               const WidthMasks& wm, uint32_t q, i128 carry, uint64_t ll_ones,
                                               SegDigits& s) {
@@ -182,7 +185,7 @@ __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, in
     }
     return ((i128)cB << 32) + (i128)cA;
   }
-  if (norm64_applies(q)) {
+  if (!GENERAL || norm64_applies(q)) {
     uint64_t c = (uint64_t)(int64_t)carry;
 #pragma unroll
     for (int i = 0; i < SEG_ELEMS; ++i) {
@@ -212,9 +215,10 @@ __device__ __forceinline__ i128 seg_normalise(const e61* L61, const e31* L31, in
 
 // Add carry c at the segment's first digit; the top digit absorbs the rest.
 // A wave skips the remaining digits once no lane has a carry left. Tests with
-// 17 <= q <= 31 have digits below 2^32 and carries in [0, 2^77) here, which
+// 16 <= q <= 31 have digits below 2^32 and carries in [0, 2^78) here, which
 // ripple in the 64-bit form of the normalisation (ripple_digit32); with
 // 32 <= q <= 43 carries are below 2^61 (ripple_digit64).
+template <bool GENERAL = true>
 __device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const WidthMasks& wm, uint32_t q, i128 c) {
   if (norm32_applies(q)) {
     uint64_t cA = (uint64_t)c & 0xFFFFFFFFull, cB = (uint64_t)(c >> 32);
@@ -224,7 +228,7 @@ __device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const
     s.d[2 * SEG_ELEMS - 1] += (int64_t)(cA + (cB << 32));
     return;
   }
-  if (norm64_applies(q)) {  // carries in [0, 2^61)
+  if (!GENERAL || norm64_applies(q)) {  // carries in [0, 2^61)
     uint64_t cc = (uint64_t)c;
 #pragma unroll
     for (int i = 0; i < 2 * SEG_ELEMS - 1; ++i)
@@ -244,7 +248,7 @@ __device__ __forceinline__ void seg_ripple(SegDigits& s, const SegMeta& M, const
 }
 
 // The segment's digits -> weighted field elements at LDS index idx0. With
-// nonneg (tests with 17 <= q <= 43) every digit is in [0, 2^61).
+// nonneg (tests with 16 <= q <= 43) every digit is in [0, 2^61).
 template <bool NONNEG = false>
 __device__ __forceinline__ void seg_weight_as(const SegDigits& s, e61* L61, e31* L31, int idx0, const SegMeta& M) {
 #pragma unroll
@@ -260,9 +264,10 @@ __device__ __forceinline__ void seg_weight_as(const SegDigits& s, e61* L61, e31*
     }
   }
 }
+template <bool GENERAL = true>
 __device__ __forceinline__ void seg_weight(const SegDigits& s, e61* L61, e31* L31, int idx0, const SegMeta& M,
                                            uint32_t q) {
-  if (norm32_applies(q) || norm64_applies(q)) seg_weight_as<true>(s, L61, L31, idx0, M);
+  if (!GENERAL || norm32_applies(q) || norm64_applies(q)) seg_weight_as<true>(s, L61, L31, idx0, M);
   else seg_weight_as<false>(s, L61, L31, idx0, M);
 }
 
@@ -1015,6 +1020,9 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
   static_assert(NSEG <= SPT * NTH, "every carry segment has a thread");
   static_assert(!WAVE || LOGH <= SMALL_WAVE_LOG_H, "one wave per field holds the transform");
   constexpr bool GTW = LOGH > SMALL_LDS_LOG_H;  // part of the twiddles read from global memory
+  // the compiled-in plans of n >= 16 (CL >= 3) only meet 16 <= q <= 42: their
+  // carry has no 128-bit form (fewer registers, less spilling at h = 4096)
+  constexpr bool GEN = CL < 3;
   constexpr int TN = GTW ? SMALL_LDS_TW : (1 << LOGH);
   constexpr int SN = GTW ? 1 : (1 << (LOGH - 1));
   // squaring pairs per thread (h/2 pairs over NTH/2 threads per field)
@@ -1067,7 +1075,7 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
   for (uint32_t it = 0; it < td.iters; ++it) {
 #pragma unroll
     for (int i = 0; i < SPT; ++i)
-      if (s0 + i < nseg) seg_weight(sd[i], L61, L31, (s0 + i) * SEG_ELEMS, M[i], D.q);
+      if (s0 + i < nseg) seg_weight<GEN>(sd[i], L61, L31, (s0 + i) * SEG_ELEMS, M[i], D.q);
     __syncthreads();
     // constant plan: steps [0, KL) from LDS, the rest (GTW) from global memory
     constexpr int NSC = CL ? small_nsteps(CL) : 0, KL = GTW ? small_lds_steps(CL, TN) : NSC;
@@ -1092,14 +1100,14 @@ __global__ void __launch_bounds__(NTH) k_small(const TestDesc* __restrict__ test
       const int s = s0 + i;
       if (s < nseg) {
         const i128 c0 = (td.mode == 0 && s == 0) ? (i128)(-2) : (i128)0;
-        store_carry(gs, s, seg_normalise(L61, L31, s * SEG_ELEMS, M[i], wm, D.q, c0, td.mode == 0 ? ~0ull : 0ull,
+        store_carry(gs, s, seg_normalise<GEN>(L61, L31, s * SEG_ELEMS, M[i], wm, D.q, c0, td.mode == 0 ? ~0ull : 0ull,
                                          sd[i]));
       }
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < SPT; ++i)
-      if (s0 + i < nseg) seg_ripple(sd[i], M[i], wm, D.q, load_carry(gs, (s0 + i + nseg - 1) % nseg));
+      if (s0 + i < nseg) seg_ripple<GEN>(sd[i], M[i], wm, D.q, load_carry(gs, (s0 + i + nseg - 1) % nseg));
   }
 #pragma unroll
   for (int i = 0; i < SPT; ++i)

@@ -222,7 +222,7 @@ int main() {
     CHECK(crt61_31(res61(hi_term), res31(hi_term)) == hi_term, "crt61_31 reads 3 2^90 + 12345 as positive");
     CHECK(crt61_31(res61(M - 1), res31(M - 1)) == -1, "M - 1 is -1");
   }
-  // segment normalisation with 64-bit words (17 <= q <= 31) against the
+  // segment normalisation with 64-bit words (16 <= q <= 31) against the
   // 128-bit reading: 8 terms in [0, 2^92 - 2^64] with random unweighting
   // rotations, residues in either representation of 0, PRP (add 0) and LL
   // (add 2^w - 1, carry-in -2 at the first digit), and positive carry-ins
@@ -232,7 +232,7 @@ int main() {
     auto res61 = [](i128 x) { return (uint64_t)(x % (i128)P61); };
     auto res31 = [](i128 x) { return (uint32_t)(x % (i128)P31); };
     for (int trial = 0; trial < N / 4; ++trial) {
-      const uint32_t q = 17u + (uint32_t)(rng() % 15);
+      const uint32_t q = 16u + (uint32_t)(rng() % 16);
       CHECK(norm32_applies(q), "norm32_applies %u", q);
       const int mode = (int)(rng() % 3);  // 0 PRP, 1 LL, 2 PRP with a positive carry-in
       const i128 cin = mode == 1 ? (rng() % 2 ? -2 : 0) : (mode == 2 ? (i128)(rng() % (1ull << 32)) : 0);
@@ -266,7 +266,7 @@ int main() {
       const i128 cout = ((i128)cB << 32) + (i128)cA;
       CHECK(cout == c, "norm_digit32 carry-out q=%u", q);
     }
-    CHECK(!norm32_applies(16) && !norm32_applies(32), "norm32_applies range");
+    CHECK(norm32_applies(16) && !norm32_applies(15) && !norm32_applies(32), "norm32_applies range");
     CHECK(norm64_applies(32) && norm64_applies(43) && !norm64_applies(31) && !norm64_applies(44),
           "norm64_applies range");
     // widths 32..44 (q = 32..43): the one-word carry against the 128-bit reading
@@ -319,7 +319,7 @@ int main() {
     // the ripple of a carry in [0, 2^77) through 7 digits below 2^w, the top
     // digit taking the rest, against 128-bit addition and shifts
     for (int trial = 0; trial < N / 4; ++trial) {
-      const uint32_t q = 17u + (uint32_t)(rng() % 15);
+      const uint32_t q = 16u + (uint32_t)(rng() % 16);
       uint64_t d[8];
       uint32_t meta[8];
       for (int i = 0; i < 8; ++i) {
