@@ -71,7 +71,7 @@ M2_FLOOR_PROBE(sq_31, e31, 2, true, square_pair(v[0], v[1], w[0]))
 // Carry of one segment (8 digits): spectrum in LDS, digits and digit
 // metadata as in the outer passes. MODE 0: skeleton, 1: unweight + CRT +
 // normalise, 2: ripple + weight, 3: both, 4: weight alone; every part takes
-// its form for digit widths 17..32 (q = 26, as in both workloads), with bit 8
+// its form for digit widths 16..32 (q = 26, as in both workloads), with bit 8
 // of MODE the one for widths 32..44 (q = 40), with bit 16 the general one
 // (q = 50).
 template <int MODE>

@@ -6,7 +6,7 @@ Compiles csrc/probes/valu_floor.hip for gfx950 (hipcc, no GPU needed) and
 counts the VALU instructions of each arithmetic primitive in the assembly: a
 radix-r butterfly with and without twiddle products in either field, the
 squaring of a spectral pair, and the carry of one 8-digit segment (unweight,
-CRT and normalisation, in the 64-bit form that digit widths 17..32 take, as
+CRT and normalisation, in the 64-bit form that digit widths 16..32 take, as
 norm64 the one of widths 32..44 and as norm128 the 128-bit form of the
 others; ripple; weight), each minus a skeleton kernel with the
 same loads and stores. A kernel's floor per wave is the sum of the

@@ -1,15 +1,16 @@
-"""GPU checks at the edges of the 64-bit carry form (digit widths 17..32).
+"""GPU checks at the edges of the 64-bit carry forms (digit widths 16..44).
 
-Tests whose digits are q or q + 1 bits wide with 17 <= q <= 31 normalise and
-ripple their carries in 64-bit words (norm_digit32, ripple_digit32 in
-m2_digits.h); q = 17 and q = 31 are its ends (w = 17 and w = 32, where the
+Tests whose digits are q or q + 1 bits wide with 16 <= q <= 31 normalise and
+ripple their carries in two 64-bit words (norm_digit32, ripple_digit32 in
+m2_digits.h), those with 32 <= q <= 43 in one (norm_digit64). q = 16 (only at
+n >= 2^25) and q = 31 are the first form's ends (w = 16 and w = 32, where the
 digit mask is all 32 bits and the carry's shift by 32 - w is 0), q = 32 the
-first width of the 128-bit form. q = 17 occurs only at n = 8 (p = 137) and at
-the bottom of the ranges of n >= 2^21, which no golden fixture reaches. Each
+second form's first width; q = 17 occurs only at n = 8 (p = 137) and at the
+bottom of the ranges of n = 2^21..2^24, which no golden fixture reaches. Each
 exponent here runs:
 
   * from the initial state, against the oracle (or, at p = 137, the whole
-    test against Python integers);
+    test against Python integers; n = 2^25 only from the residues below);
   * from near-maximal residues, whose results the mathematics fixes:
     PRP from M_p - 3 gives 9, 81, 6561; LL from M_p - 3 gives 7, 47, 2207 and
     from M_p - 1 stays at M_p - 1.
@@ -21,7 +22,8 @@ import mersenne2gpu_amd as m2
 pytestmark = pytest.mark.gpu
 
 # (p, log2 n, q = p >> log2 n)
-EDGES = [(137, 3, 17), (262139, 13, 31), (33554393, 20, 31), (33554467, 20, 32), (36700201, 21, 17)]
+EDGES = [(137, 3, 17), (262139, 13, 31), (33554393, 20, 31), (33554467, 20, 32), (36700201, 21, 17),
+         (553648171, 25, 16)]
 
 
 def _r64(x):
@@ -42,7 +44,7 @@ def test_widths_at_the_edges(gpu_ctx, p, ln, q):
                 v = v * v - c if d != 1 else v
 
 
-@pytest.mark.parametrize("p,ln,q", [e for e in EDGES if e[0] > 1000])
+@pytest.mark.parametrize("p,ln,q", [e for e in EDGES if 1000 < e[0] < 10 ** 8])
 def test_widths_against_the_oracle(gpu_ctx, oracle, p, ln, q):
     ks = [40, 60]
     for mode in ("prp", "ll"):
