@@ -51,9 +51,9 @@ PLANS = {
 
 # SQ_INSTS_VALU per wave, rocprofv3 (profiles named above)
 ISSUED = {
-    "k_centre<1024, 10>": 2402, "k_outer_fused<9, 2, 10>": 3237, "k_outer_fwd<9, 2, 10>": 1538,
-    "k_outer_inv<9, 2, 10>": 1946,
-    "k_centre<512, 10>": 4501, "k_outer_inv<7, 4, 16>": 1925, "k_outer_fwd<7, 4, 16>": 1469,
+    "k_centre<1024, 10>": 2402, "k_outer_fused<9, 2, 10>": 3204, "k_outer_fwd<9, 2, 10>": 1538,
+    "k_outer_inv<9, 2, 10>": 1904,
+    "k_centre<512, 10>": 4501, "k_outer_inv<7, 4, 16>": 1893, "k_outer_fwd<7, 4, 16>": 1466,
     "k_mid<true, 6, 5, 10>": 1083, "k_mid<false, 6, 5, 10>": 1091,
 }
 
