@@ -13,11 +13,15 @@ Prints one JSON line. Extra fields:
                 duration (HIP events on the test's stream) against 8 TB/s;
                 traffic = measured HBM bytes per launch from the committed
                 rocprofv3 counter summary named in traffic_source;
+                traffic_sources_match says whether the kernel sources
+                that summary was collected on (a digest it records) are
+                the ones this run was built from;
   int_roofline  the integer co-bound: the iteration's algorithmic base products
                 (49.5 h per field; 4 v_mad_u64_u32 per M61 product, 1 per M31
                 product) per second against the device's v_mad_u64_u32 issue
                 rate measured in the same run, and the dominant kernel's VALU
-                busy fraction per SIMD from the committed counter summary;
+                busy fraction per SIMD from the committed counter summary
+                (valu_busy_sources_match: the same digest check);
   config1       the n = 512 workload: whole PRP tests (the seeded exponent
                 in [10499, 20983] and 19937) to completion, one at a time,
                 final Res64 against the exact fixture, seconds per test;
@@ -42,6 +46,7 @@ Prints one JSON line. Extra fields:
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -376,6 +381,33 @@ def served_leg(ctx, seed: int, count: int, iters: int, budget_gb: float, slice_i
     return out
 
 
+# the sources whose change can change a kernel's counters: the device code and
+# the host code that plans launches and lays out tables
+KERNEL_SOURCES = ("m2_kernels.hip", "m2_arith.h", "m2_digits.h", "m2_plan.h", "m2_internal.h", "m2_host.cpp",
+                  "m2_api.cpp")
+
+
+def kernel_sources_digest():
+    """SHA-256 (first 16 hex digits) over the kernel and planner sources, as
+    recorded in the committed counter summaries' `_sources_sha256`."""
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "mersenne2gpu_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def _profile_matches(name):
+    """True when a committed counter summary was collected on the sources this
+    run was built from, False when they differ, None when it records none."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            rec = json.load(f).get("_sources_sha256")
+        return None if rec is None else rec == kernel_sources_digest()
+    except (OSError, ValueError):
+        return None
+
+
 def _profile_source(name):
     """The `_source` note of a committed counter summary (which profile and
     build its figures come from: they are not measured by this run)."""
@@ -559,6 +591,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,
+                     "traffic_sources_match": _profile_matches("traffic.json") if traffic is not None else None,
                      "bytes_per_launch": bytes_dom, "us_per_launch": round(per_kernel_us[dom], 3)},
         "int_roofline": {"bound": "v_mad_u64_u32 issue", "unit": "T products/s",
                          "achieved": round(products / (us_per_iter * 1e-6) / 1e12, 3),
@@ -568,7 +601,8 @@ def main() -> int:
                          "modmuls_per_iter": int(modmuls),
                          "achieved_gmodmul_s": round(modmuls / (us_per_iter * 1e-6) / 1e9, 1),
                          "kernel": dom, "valu_busy": load_valu(dom, wl["traffic_key"]),
-                         "valu_busy_source": _profile_source("valu.json")},
+                         "valu_busy_source": _profile_source("valu.json"),
+                         "valu_busy_sources_match": _profile_matches("valu.json")},
         "config1": config1,
         "config3": config3,
         "batched": batched,
